@@ -1,0 +1,51 @@
+"""Build the HIP library libmfhip.so in-tree for gfx950 with hipcc.
+
+Packed FP32 instructions are disabled for all device code
+(``-target-feature -packed-fp32-ops``); the kernels are written in scalar
+FP32 and compiled for CDNA4 only.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmfhip.so")
+
+SOURCES = ["mfh_api.hip", "mfh_flat.hip"]
+ARCH = os.environ.get("MFH_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found; the HIP library cannot be built")
+
+
+def command(out=LIB):
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    return [hipcc(), "--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+            "-o", out] + srcs
+
+
+def build(verbose=False):
+    """Compile every source into libmfhip.so (always recompiles)."""
+    tmp = LIB + ".tmp%d" % os.getpid()
+    cmd = command(tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed (exit %d)" % res.returncode)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

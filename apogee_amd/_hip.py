@@ -1,0 +1,205 @@
+"""ctypes binding of libmfhip.so and the device Context used by the filters.
+
+The Context plays the role of the reference's backend Context protocol
+(python/matchedfilter/_vkcompute.py: ``peaks`` :962-1063, ``hier_peaks``
+:534-623, ``forward`` :855-906): the public classes call it at exactly the
+places the reference calls ``self._gpu.peaks`` / ``.hier_peaks`` /
+``.forward``.  Unlike the reference it keeps the spectra on the device from
+the moment they are set, so nothing is re-uploaded per run.
+
+There is no fallback: if the HIP library is missing or no HIP device is
+visible, constructing a Context raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+from ._errors import UnsupportedSize
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmfhip.so")
+
+_lib = None
+_lock = threading.Lock()
+
+_c_size = ctypes.c_size_t
+_vp = ctypes.c_void_p
+
+
+def _declare(lib):
+    lib.mfh_last_error.restype = ctypes.c_char_p
+    lib.mfh_device_count.restype = ctypes.c_int
+    lib.mfh_device_info.argtypes = [ctypes.c_int, ctypes.c_char_p, _c_size, ctypes.c_char_p, _c_size]
+    lib.mfh_supported.argtypes = [_c_size]
+    lib.mfh_create.restype = _vp
+    lib.mfh_create.argtypes = [ctypes.c_int, _c_size, ctypes.c_int, ctypes.c_int]
+    lib.mfh_destroy.argtypes = [_vp]
+    lib.mfh_destroy.restype = None
+    lib.mfh_stream.argtypes = [_vp]
+    lib.mfh_stream.restype = _vp
+    lib.mfh_sync.argtypes = [_vp]
+    for name in ("mfh_set_data", "mfh_set_templates", "mfh_set_data_device_async",
+                 "mfh_set_templates_device_async"):
+        getattr(lib, name).argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp]
+    lib.mfh_nbins.restype = _c_size
+    lib.mfh_nbins.argtypes = [_c_size, _c_size, _c_size, _c_size]
+    lib.mfh_flat_peaks.restype = ctypes.c_long
+    lib.mfh_flat_peaks.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   _c_size, ctypes.c_float, _c_size, _c_size, _vp, _vp, _vp]
+    lib.mfh_flat_peaks_async.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         _c_size, ctypes.c_float, _c_size, _c_size, _vp, _vp]
+    lib.mfh_set_tuning.argtypes = [_vp, ctypes.c_char_p, ctypes.c_long]
+    lib.mfh_prof_enable.argtypes = [_vp, ctypes.c_int]
+    lib.mfh_prof_read.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_long)]
+    return lib
+
+
+def lib():
+    """The loaded HIP library; raises RuntimeError if it was never built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    "the HIP library %s is missing: build it with "
+                    "`python -m apogee_amd._build` (hipcc, gfx950)" % LIB_PATH)
+            _lib = _declare(ctypes.CDLL(LIB_PATH))
+        return _lib
+
+
+def last_error():
+    return lib().mfh_last_error().decode(errors="replace")
+
+
+def check(status, what="mfh"):
+    if status < 0:
+        raise RuntimeError("%s failed: %s" % (what, last_error()))
+    return status
+
+
+def device_count():
+    return int(lib().mfh_device_count())
+
+
+def device_info(index):
+    name = ctypes.create_string_buffer(256)
+    arch = ctypes.create_string_buffer(64)
+    check(lib().mfh_device_info(int(index), name, 256, arch, 64), "mfh_device_info")
+    return name.value.decode(), arch.value.decode()
+
+
+def supported(n):
+    return bool(lib().mfh_supported(int(n)))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _device_pointer(a):
+    """Device address of a ROCm tensor (torch), or None for host arrays."""
+    if hasattr(a, "data_ptr") and getattr(a, "is_cuda", False):
+        return int(a.data_ptr())
+    return None
+
+
+class Context:
+    """Device state of one filter: spectra resident in HBM plus a HIP stream.
+
+    ``n``, ``ndata`` and ``ntemplates`` fix the device allocation; rows are
+    written by :meth:`set_rows` and read by every run until overwritten.
+    """
+
+    def __init__(self, index, n, ndata, ntemplates):
+        L = lib()
+        if not supported(n):
+            raise UnsupportedSize("transform length %d is not supported on this device" % n)
+        self.index = int(index)
+        self.n, self.ndata, self.ntemplates = int(n), int(ndata), int(ntemplates)
+        h = L.mfh_create(self.index, self.n, self.ndata, self.ntemplates)
+        if not h:
+            raise RuntimeError("mfh_create failed: %s" % last_error())
+        self._h = ctypes.c_void_p(h)
+        self.last_refinements = 0
+
+    # ------------------------------------------------------------ lifetime
+    def destroy(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.mfh_destroy(h)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise RuntimeError("context destroyed")
+        return self._h
+
+    def stream(self):
+        return int(lib().mfh_stream(self.handle) or 0)
+
+    def sync(self):
+        check(lib().mfh_sync(self.handle), "mfh_sync")
+
+    # -------------------------------------------------------------- ingest
+    def set_rows(self, kind, row0, spectra):
+        """Copy spectra into data (kind="data") or template rows from row0.
+
+        ``spectra`` is a C-contiguous complex64 host array (rows, n), or a
+        complex64 ROCm tensor of the same shape (copied device to device).
+        """
+        dev = _device_pointer(spectra)
+        rows = int(spectra.shape[0])
+        L = lib()
+        if dev is not None:
+            fn = L.mfh_set_data_device_async if kind == "data" else L.mfh_set_templates_device_async
+            check(fn(self.handle, int(row0), rows, ctypes.c_void_p(dev)), "set %s" % kind)
+            self.sync()
+            return
+        fn = L.mfh_set_data if kind == "data" else L.mfh_set_templates
+        check(fn(self.handle, int(row0), rows, _ptr(spectra)), "set %s" % kind)
+
+    # ---------------------------------------------------------------- runs
+    def peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, counts=False):
+        """Flat filter over rows [d0, d0+nd) x [t0, t0+nt); host results.
+
+        Returns ``(index int64, value complex64)`` of shape (nd, nt, nbins), and
+        the per-pair crossing counts (int32, (nd, nt)) when ``counts``.
+        """
+        nb = int(lib().mfh_nbins(self.n, int(binsize), int(start), int(end)))
+        idx = np.empty((nd, nt, nb), dtype=np.int64)
+        val = np.empty((nd, nt, nb), dtype=np.complex64)
+        cnt = np.empty((nd, nt), dtype=np.int32) if counts else None
+        check(lib().mfh_flat_peaks(self.handle, int(d0), int(nd), int(t0), int(nt), int(binsize),
+                                   float(threshold), int(start), int(end), _ptr(idx), _ptr(val),
+                                   _ptr(cnt) if cnt is not None else None), "mfh_flat_peaks")
+        return idx, val, cnt
+
+    def peaks_async(self, d0, nd, t0, nt, binsize, threshold, start, end, idx_ptr, val_ptr):
+        """Queue a flat run writing device buffers (int64 / complex64 pointers)."""
+        check(lib().mfh_flat_peaks_async(self.handle, int(d0), int(nd), int(t0), int(nt), int(binsize),
+                                         float(threshold), int(start), int(end),
+                                         ctypes.c_void_p(int(idx_ptr)), ctypes.c_void_p(int(val_ptr))),
+              "mfh_flat_peaks_async")
+
+    # ------------------------------------------------------------- tuning
+    def set_tuning(self, key, value):
+        check(lib().mfh_set_tuning(self.handle, key.encode(), int(value)), "mfh_set_tuning")
+
+    def prof_enable(self, on=True):
+        check(lib().mfh_prof_enable(self.handle, 1 if on else 0), "mfh_prof_enable")
+
+    def prof_read(self, name):
+        ms = ctypes.c_double(0.0)
+        cnt = ctypes.c_long(0)
+        check(lib().mfh_prof_read(self.handle, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)),
+              "mfh_prof_read")
+        return ms.value, cnt.value

@@ -1,0 +1,297 @@
+// C ABI of the MI355X matched filter: contexts, ingest, dispatch, timing.
+// Declarations and the reference interface each replaces: include/mfhip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mfhip.h"
+#include "mfh_flat.h"
+#include "mfh_internal.h"
+
+using mfh::cf;
+
+namespace {
+thread_local std::string g_err;
+}
+
+namespace mfh {
+
+int fail(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+bool hip_ok(hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  fail("%s: %s", what, hipGetErrorString(e));
+  return false;
+}
+
+Ctx::~Ctx() {
+  if (dev >= 0) hipSetDevice(dev);
+  for (auto &kv : bufs) hipFree(kv.second.p);
+  for (auto &e : prof_events) {
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  if (stream) hipStreamDestroy(stream);
+}
+
+void *Ctx::scratch(const char *name, size_t bytes) {
+  Buf &b = bufs[name];
+  if (b.bytes >= bytes && b.p) return b.p;
+  if (b.p) {
+    hipStreamSynchronize(stream);
+    hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (!hip_ok(hipMalloc(&b.p, bytes ? bytes : 1), "hipMalloc")) return nullptr;
+  b.bytes = bytes;
+  return b.p;
+}
+
+// Event pair around one launch, when profiling is on.
+Ctx::ProfScope::ProfScope(Ctx *c, const char *name) : c_(c), name_(name) {
+  if (!c_->prof_on) return;
+  hipEventCreate(&ev_.a);
+  hipEventCreate(&ev_.b);
+  hipEventRecord(ev_.a, c_->stream);
+}
+Ctx::ProfScope::~ProfScope() {
+  if (!c_->prof_on) return;
+  hipEventRecord(ev_.b, c_->stream);
+  ev_.name = name_;
+  c_->prof_events.push_back(ev_);
+}
+
+void Ctx::prof_collect() {
+  if (prof_events.empty()) return;
+  hipStreamSynchronize(stream);
+  for (auto &e : prof_events) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e.a, e.b);
+    auto &acc = prof_totals[e.name];
+    acc.first += ms;
+    acc.second += 1;
+    hipEventDestroy(e.a);
+    hipEventDestroy(e.b);
+  }
+  prof_events.clear();
+}
+
+}  // namespace mfh
+
+using mfh::Ctx;
+using mfh::fail;
+using mfh::hip_ok;
+
+// ---------------------------------------------------------------- the ABI
+
+extern "C" const char *mfh_last_error(void) { return g_err.c_str(); }
+
+extern "C" int mfh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+extern "C" int mfh_device_info(int index, char *name, size_t name_len, char *arch, size_t arch_len) {
+  hipDeviceProp_t p;
+  if (!hip_ok(hipGetDeviceProperties(&p, index), "hipGetDeviceProperties")) return -1;
+  if (name && name_len) snprintf(name, name_len, "%s", p.name);
+  if (arch && arch_len) snprintf(arch, arch_len, "%s", p.gcnArchName);
+  return 0;
+}
+
+extern "C" int mfh_supported(size_t n) { return mfh::flat_supported(n) ? 1 : 0; }
+
+extern "C" mfh_ctx *mfh_create(int device, size_t n, int nd, int nt) {
+  if (!mfh::flat_supported(n)) {
+    fail("transform length %zu is not supported", n);
+    return nullptr;
+  }
+  if (nd < 1 || nt < 1) {
+    fail("ndata and ntemplates must be >= 1");
+    return nullptr;
+  }
+  int count = mfh_device_count();
+  if (device < 0 || device >= count) {
+    fail("no HIP device with index %d (%d visible)", device, count);
+    return nullptr;
+  }
+  if (!hip_ok(hipSetDevice(device), "hipSetDevice")) return nullptr;
+  Ctx *c = new Ctx;
+  c->dev = device;
+  c->n = n;
+  c->nd = nd;
+  c->nt = nt;
+  if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) {
+    delete c;
+    return nullptr;
+  }
+  c->data = (cf *)c->scratch("data", (size_t)nd * n * sizeof(cf));
+  c->tmpl = (cf *)c->scratch("tmpl", (size_t)nt * n * sizeof(cf));
+  c->tw = (cf *)c->scratch("twiddle", (mfh::flat_twiddle_count(n) + 1) * sizeof(cf));
+  if (!c->data || !c->tmpl || !c->tw) {
+    delete c;
+    return nullptr;
+  }
+  // unset rows read as zero rather than as whatever the allocator left
+  hipMemsetAsync(c->data, 0, (size_t)nd * n * sizeof(cf), c->stream);
+  hipMemsetAsync(c->tmpl, 0, (size_t)nt * n * sizeof(cf), c->stream);
+  if (mfh::flat_fill_twiddles(c) < 0 || !hip_ok(hipStreamSynchronize(c->stream), "create")) {
+    delete c;
+    return nullptr;
+  }
+  return reinterpret_cast<mfh_ctx *>(c);
+}
+
+extern "C" void mfh_destroy(mfh_ctx *ctx) { delete reinterpret_cast<Ctx *>(ctx); }
+
+extern "C" void *mfh_stream(mfh_ctx *ctx) {
+  return ctx ? (void *)reinterpret_cast<Ctx *>(ctx)->stream : nullptr;
+}
+
+extern "C" int mfh_sync(mfh_ctx *ctx) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  hipSetDevice(c->dev);
+  return hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize") ? 0 : -1;
+}
+
+static int set_rows(Ctx *c, cf *base, int total, int row0, int rows, const void *src, hipMemcpyKind kind,
+                    bool sync) {
+  if (row0 < 0 || rows < 1 || row0 + rows > total) return fail("rows [%d, %d) out of range [0, %d)", row0, row0 + rows, total);
+  if (!src) return fail("null spectrum pointer");
+  hipSetDevice(c->dev);
+  if (!hip_ok(hipMemcpyAsync(base + (size_t)row0 * c->n, src, (size_t)rows * c->n * sizeof(cf), kind, c->stream),
+              "hipMemcpyAsync"))
+    return -1;
+  if (sync && !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return -1;
+  return 0;
+}
+
+extern "C" int mfh_set_data(mfh_ctx *ctx, int row0, int rows, const void *host_c64) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  return set_rows(c, c->data, c->nd, row0, rows, host_c64, hipMemcpyHostToDevice, true);
+}
+extern "C" int mfh_set_templates(mfh_ctx *ctx, int row0, int rows, const void *host_c64) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  return set_rows(c, c->tmpl, c->nt, row0, rows, host_c64, hipMemcpyHostToDevice, true);
+}
+extern "C" int mfh_set_data_device_async(mfh_ctx *ctx, int row0, int rows, const void *dev_c64) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  return set_rows(c, c->data, c->nd, row0, rows, dev_c64, hipMemcpyDeviceToDevice, false);
+}
+extern "C" int mfh_set_templates_device_async(mfh_ctx *ctx, int row0, int rows, const void *dev_c64) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  return set_rows(c, c->tmpl, c->nt, row0, rows, dev_c64, hipMemcpyDeviceToDevice, false);
+}
+
+extern "C" size_t mfh_nbins(size_t n, size_t binsize, size_t ws, size_t we) {
+  if (!binsize) return 0;
+  if (we > n) we = n;
+  if (ws >= we) return 0;
+  return (we - ws + binsize - 1) / binsize;
+}
+
+extern "C" int mfh_flat_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
+                                    size_t ws, size_t we, int64_t *dev_idx, void *dev_val_c64) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  mfh::FlatRun r;
+  r.data = c->data + (size_t)d0 * c->n;
+  r.tmpl = c->tmpl + (size_t)t0 * c->n;
+  r.data_stride = (long long)c->n;
+  r.tmpl_stride = (long long)c->n;
+  if (d0 < 0 || t0 < 0 || nd < 1 || nt < 1 || d0 + nd > c->nd || t0 + nt > c->nt)
+    return fail("data/templates sub-range out of bounds");
+  r.nd = nd;
+  r.nt = nt;
+  r.binsize = binsize;
+  r.threshold = threshold;
+  r.ws = ws;
+  r.we = we;
+  r.out_idx = dev_idx;
+  r.out_val = (cf *)dev_val_c64;
+  r.out_row_stride = 0;
+  return mfh::launch_flat(c, r);
+}
+
+extern "C" long mfh_flat_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
+                               size_t ws, size_t we, int64_t *idx, void *val_c64, int32_t *counts) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!idx || !val_c64) return fail("null output pointer");
+  const size_t nb = mfh_nbins(c->n, binsize, ws, we);
+  if (binsize < 1) return fail("binsize must be >= 1");
+  if (nb == 0) return fail("empty window (%zu, %zu)", ws, we);
+  const size_t cells = (size_t)nd * (size_t)nt * nb;
+  hipSetDevice(c->dev);
+  int64_t *didx = (int64_t *)c->scratch("out_idx", cells * sizeof(int64_t));
+  cf *dval = (cf *)c->scratch("out_val", cells * sizeof(cf));
+  if (!didx || !dval) return -1;
+  if (mfh_flat_peaks_async(ctx, d0, nd, t0, nt, binsize, threshold, ws, we, didx, dval) < 0) return -1;
+  if (!hip_ok(hipMemcpyAsync(idx, didx, cells * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream), "copy idx") ||
+      !hip_ok(hipMemcpyAsync(val_c64, dval, cells * sizeof(cf), hipMemcpyDeviceToHost, c->stream), "copy val") ||
+      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+    return -1;
+  c->prof_collect();
+  long total = 0;
+  for (size_t p = 0; p < (size_t)nd * nt; p++) {
+    int cnt = 0;
+    for (size_t b = 0; b < nb; b++) cnt += idx[p * nb + b] >= 0;
+    if (counts) counts[p] = cnt;
+    total += cnt;
+  }
+  return total;
+}
+
+extern "C" int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value) {
+  if (!ctx || !key) return fail("null argument");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!strcmp(key, "dpw")) {
+    if (value < 0) return fail("dpw must be >= 0");
+    c->tune_dpw = (int)value;
+    return 0;
+  }
+  return fail("unknown tuning key '%s'", key);
+}
+
+extern "C" int mfh_prof_enable(mfh_ctx *ctx, int on) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  c->prof_collect();
+  c->prof_on = on != 0;
+  c->prof_totals.clear();
+  return 0;
+}
+
+extern "C" int mfh_prof_read(mfh_ctx *ctx, const char *name, double *total_ms, long *count) {
+  if (!ctx || !name) return fail("null argument");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  c->prof_collect();
+  auto it = c->prof_totals.find(name);
+  if (total_ms) *total_ms = it == c->prof_totals.end() ? 0.0 : it->second.first;
+  if (count) *count = it == c->prof_totals.end() ? 0 : it->second.second;
+  return 0;
+}

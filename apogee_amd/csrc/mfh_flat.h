@@ -1,0 +1,332 @@
+// Fused flat matched filter for gfx950: D*conj(T) -> inverse FFT -> per-bin
+// |z| argmax, with the correlation series never leaving registers and LDS.
+//
+// Reference behaviour (paths relative to the reference checkout):
+//   product / transform / scan   src/balanced-inl.h:1102-1113, :876-969
+//   selection rule               src/balanced-inl.h:886, :944, :956
+//   GPU precedent (one transform per workgroup, LDS exchange, one elected
+//   writer per bin)              src/gpu/tierb.slang:591-771
+//
+// Layout.  A transform of length N is carried by TPT = N/16 threads holding
+// 16 points each: thread t owns elements t + u*TPT, u = 0..15.  The inverse
+// FFT is a Stockham autosort over radix passes [2^b] + [16]*L (N = 2^b 16^L),
+// every pass a register DFT preceded by twiddles from an FP64-generated table
+// and followed by a padded LDS exchange; after the last pass thread t holds
+// z[t + u*TPT].  A workgroup carries SLOTS = WG/TPT transforms side by side;
+// each slot keeps one template spectrum in registers and walks a tile of data
+// rows, so every template is read from HBM once per tile and every data row
+// is reused from L2 across the template bank.
+//
+// Selection.  Each lag's key is (bits(|z|^2) << 32) | ~k with
+// |z|^2 = fma(re, re, im*im) in FP32; only keys with |z|^2 > seed
+// (seed = threshold^2, or 0) enter.  The per-bin maximum of the keys is the
+// largest |z|^2 with the LOWEST lag among equals, independent of the order in
+// which lanes and waves contribute: per-thread runs and wave64 shuffles reduce
+// first and LDS ds_max_u64 finishes.  The lane that owns the winning lag then
+// writes index and value, so the value always belongs to the reported lag.
+#pragma once
+#include "mfh_dft.h"
+
+namespace mfh {
+
+struct FlatArgs {
+  const cf *data;      // [rows][N] natural-order spectra (already offset to the first run row)
+  const cf *tmpl;      // [rows][N] (already offset to the first run template)
+  const cf *twiddle;   // per-pass tables, TwiddleLayout<N>
+  int64_t *out_idx;    // [nd][nt][nb]
+  cf *out_val;         // [nd][nt][nb]
+  long long data_stride, tmpl_stride;  // elements between rows
+  int nd, nt;          // run extent
+  int dpw;             // data rows per workgroup
+  uint32_t ws, we;     // lag window [ws, we)
+  uint32_t binsize, nb;  // nb: bins of this launch
+  uint32_t ostride;      // output elements between pairs (>= nb)
+  int bshift;          // log2(binsize) when a power of two, else -1
+  float inv_bs;        // 1 / binsize
+  float seed;          // threshold^2 (FP32), or 0
+};
+
+template <int N>
+struct FlatShape {
+  static constexpr int R = 16;
+  static constexpr int TPT = N / 16;
+  static constexpr int WG = TPT >= 256 ? TPT : 256;
+  static constexpr int SLOTS = WG / TPT;
+  static constexpr int LOGN = ilog2(N);
+  static constexpr int FIRST = 1 << (LOGN % 4);  // radix of the first pass (1: none)
+  static constexpr int NPASS16 = LOGN / 4;
+  static constexpr int XCH = N + N / 16;          // padded exchange elements per slot
+};
+
+__device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
+
+__device__ __forceinline__ uint32_t bin_of(uint32_t k, const FlatArgs &a) {
+  const uint32_t x = k - a.ws;
+  if (a.bshift >= 0) return x >> a.bshift;
+  uint32_t q = (uint32_t)((float)x * a.inv_bs);
+  if (q * a.binsize > x) q -= 1;
+  else if ((q + 1) * a.binsize <= x) q += 1;
+  return q;
+}
+
+__device__ __forceinline__ uint64_t make_key(cf z, uint32_t k, float seed) {
+  const float m2 = __builtin_fmaf(z.x, z.x, z.y * z.y);
+  return m2 > seed ? ((uint64_t)__float_as_uint(m2) << 32) | (uint64_t)(~k) : 0ull;
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Twiddle tables, one per twiddled pass, laid out [k][r] so that a thread's R
+// twiddles sit at one base address plus compile-time offsets:
+//   tw[pass_offset(NS) + k*R + r] = exp(+2 pi i r k / (R NS)),  k < NS, r < R.
+// Passes with NS = 1 need none.  Generated in FP64 on the device.
+template <int N>
+struct TwiddleLayout {
+  using S = FlatShape<N>;
+  static constexpr int NS0 = S::FIRST > 1 ? S::FIRST : 1;
+  // radix-16 passes with NS = NS0 * 16^p, p = 0..NPASS16-1
+  static constexpr int offset(int p) {
+    int off = 0, ns = NS0;
+    for (int i = 0; i < p; i++) {
+      if (ns > 1) off += ns * 16;
+      ns *= 16;
+    }
+    return off;
+  }
+  static constexpr int total() { return offset(S::NPASS16); }
+};
+
+// One Stockham pass of radix R: butterflies j = t + q*TPT (q < 16/R) read
+// x[j + r*N/R] = registers v[q + r*Q], twiddle by exp(+2 pi i r (j mod NS) /
+// (R NS)) and run the R-point DFT in place.
+template <int N, int R, int NS>
+__device__ __forceinline__ void stockham_pass(cf *v, int t, const cf *tw) {
+  constexpr int TPT = N / 16;
+  constexpr int Q = 16 / R;  // butterflies per thread
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    cf b[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) b[r] = v[q + r * Q];
+    if constexpr (NS > 1) {
+      const int k = (t + q * TPT) % NS;
+      const cf *w = tw + k * R;
+#pragma unroll
+      for (int r = 1; r < R; r++) b[r] = cmul(b[r], w[r]);
+    }
+    dftR<R>(b);
+#pragma unroll
+    for (int r = 0; r < R; r++) v[q + r * Q] = b[r];
+  }
+}
+
+// Store the pass outputs y[(j/NS) NS R + (j mod NS) + r NS] to the padded LDS
+// buffer (index i lives at i + i/16: conflict-free for both the strided
+// stores and the unit-stride loads).  Base address + compile-time offsets
+// whenever the padding is separable.
+template <int N, int R, int NS>
+__device__ __forceinline__ void stockham_store(const cf *v, int t, cf *xch) {
+  constexpr int TPT = N / 16;
+  constexpr int Q = 16 / R;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int j = t + q * TPT;
+    const int base = (j / NS) * NS * R + (j % NS);
+    if constexpr (NS % 16 == 0) {
+      cf *p = xch + lds_pad(base);
+#pragma unroll
+      for (int r = 0; r < R; r++) p[r * (NS + NS / 16)] = v[q + r * Q];
+    } else if constexpr (NS == 1 && R == 16) {
+      cf *p = xch + 17 * j;
+#pragma unroll
+      for (int r = 0; r < R; r++) p[r] = v[q + r * Q];
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) xch[lds_pad(base + r * NS)] = v[q + r * Q];
+    }
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void stockham_load(cf *v, int t, const cf *xch) {
+  constexpr int TPT = N / 16;
+  if constexpr (TPT % 16 == 0) {
+    const cf *p = xch + lds_pad(t);
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = p[u * (TPT + TPT / 16)];
+  } else {
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = xch[lds_pad(t + u * TPT)];
+  }
+}
+
+// Full inverse FFT of the slot's N points held in v; returns with thread t
+// holding z[t + u*TPT].  Two barriers per exchange.
+template <int N>
+__device__ __forceinline__ void inverse_fft(cf *v, int t, cf *xch, const cf *tw) {
+  using S = FlatShape<N>;
+  using L = TwiddleLayout<N>;
+  constexpr int F = S::FIRST;
+  if constexpr (F > 1) {
+    stockham_pass<N, F, 1>(v, t, tw);
+    stockham_store<N, F, 1>(v, t, xch);
+    __syncthreads();
+    stockham_load<N>(v, t, xch);
+    __syncthreads();
+  }
+  constexpr int NS0 = L::NS0;
+  stockham_pass<N, 16, NS0>(v, t, tw + L::offset(0));
+  if constexpr (S::NPASS16 >= 2) {
+    stockham_store<N, 16, NS0>(v, t, xch);
+    __syncthreads();
+    stockham_load<N>(v, t, xch);
+    __syncthreads();
+    stockham_pass<N, 16, NS0 * 16>(v, t, tw + L::offset(1));
+  }
+  if constexpr (S::NPASS16 >= 3) {
+    stockham_store<N, 16, NS0 * 16>(v, t, xch);
+    __syncthreads();
+    stockham_load<N>(v, t, xch);
+    __syncthreads();
+    stockham_pass<N, 16, NS0 * 256>(v, t, tw + L::offset(2));
+  }
+  static_assert(S::NPASS16 >= 1 && S::NPASS16 <= 3, "N range");
+}
+
+template <int N>
+__global__ void __launch_bounds__(FlatShape<N>::WG) __attribute__((amdgpu_waves_per_eu(4)))
+flat_kernel(FlatArgs a) {
+  using S = FlatShape<N>;
+  constexpr int TPT = S::TPT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cf *xch_all = reinterpret_cast<cf *>(smem);
+  uint64_t *bins_all = reinterpret_cast<uint64_t *>(xch_all + S::SLOTS * S::XCH);
+
+  const int slot = threadIdx.x / TPT;
+  const int t0 = threadIdx.x % TPT;
+  const int lane = threadIdx.x & 63;
+  cf *xch = xch_all + slot * S::XCH;
+  const int nb = (int)a.nb;
+
+  const int tt = blockIdx.x * S::SLOTS + slot;
+  const bool tvalid = tt < a.nt;
+  const int dbeg = blockIdx.y * a.dpw;
+  const int dend = min(a.nd, dbeg + a.dpw);
+
+  // the template stays in registers for the whole data tile
+  cf T[16];
+  {
+    const cf *tp = a.tmpl + (long long)(tvalid ? tt : 0) * a.tmpl_stride;
+#pragma unroll
+    for (int u = 0; u < 16; u++) T[u] = tvalid ? tp[t0 + u * TPT] : cf{0.f, 0.f};
+  }
+  // both bin buffers start empty
+  for (int i = threadIdx.x; i < 2 * S::SLOTS * nb; i += S::WG) bins_all[i] = 0ull;
+  __syncthreads();
+
+  for (int d = dbeg, it = 0; d < dend; d++, it++) {
+    // Everything derived from the thread index (bins, window flags, LDS and
+    // twiddle addresses) is loop-invariant; recomputing it per pair is far
+    // cheaper than the registers the hoisted copies would pin.
+    int t = t0;
+    asm volatile("" : "+v"(t));
+    uint64_t *bins = bins_all + ((it & 1) * S::SLOTS + slot) * nb;
+    uint64_t *bins_next = bins_all + (((it + 1) & 1) * S::SLOTS + slot) * nb;
+    cf v[16];
+    {
+      const cf *dp = a.data + (long long)d * a.data_stride;
+#pragma unroll
+      for (int u = 0; u < 16; u++) v[u] = cmul_conj(dp[t + u * TPT], T[u]);
+    }
+    inverse_fft<N>(v, t, xch, a.twiddle);
+
+    // ---- per-bin maximum of the keys ---------------------------------
+    if constexpr (TPT >= 64) {
+      uint64_t acc = 0ull;
+      int acc_bin = -1;
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const uint32_t k = (uint32_t)(t + u * TPT);
+        const uint32_t kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
+        const uint64_t key = (k >= a.ws && k < a.we) ? make_key(v[u], k, a.seed) : 0ull;
+        const bool uni = kf >= a.ws && kf + 63 < a.we && bin_of(kf, a) == bin_of(kf + 63, a);
+        if (uni) {
+          const int b = (int)bin_of(kf, a);
+          if (b != acc_bin) {
+            if (acc_bin >= 0) {
+              const uint64_t w = wave_max_u64(acc);
+              if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
+            }
+            acc = 0ull;
+            acc_bin = b;
+          }
+          acc = key > acc ? key : acc;
+        } else if (key) {
+          atomicMax((unsigned long long *)&bins[bin_of(k, a)], (unsigned long long)key);
+        }
+      }
+      if (acc_bin >= 0) {
+        const uint64_t w = wave_max_u64(acc);
+        if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
+      }
+    } else {
+      // several transforms share a wave: per-thread runs, then LDS atomics
+      uint64_t acc = 0ull;
+      int acc_bin = -1;
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const uint32_t k = (uint32_t)(t + u * TPT);
+        if (k >= a.ws && k < a.we) {
+          const uint64_t key = make_key(v[u], k, a.seed);
+          const int b = (int)bin_of(k, a);
+          if (b != acc_bin) {
+            if (acc) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)acc);
+            acc = 0ull;
+            acc_bin = b;
+          }
+          acc = key > acc ? key : acc;
+        }
+      }
+      if (acc) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)acc);
+    }
+    __syncthreads();
+
+    // ---- the owner of each winning lag writes it; empty bins get -1/0 ----
+    if (tvalid) {
+      const long long row = (long long)d * a.nt + tt;
+      int64_t *oidx = a.out_idx + row * a.ostride;
+      cf *oval = a.out_val + row * a.ostride;
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const uint32_t k = (uint32_t)(t + u * TPT);
+        if (k >= a.ws && k < a.we) {
+          const uint64_t key = make_key(v[u], k, a.seed);
+          if (key) {
+            const uint32_t b = bin_of(k, a);
+            if (bins[b] == key) {
+              oidx[b] = (int64_t)k;
+              oval[b] = v[u];
+            }
+          }
+        }
+      }
+      for (int b = t; b < nb; b += TPT) {
+        if (bins[b] == 0ull) {
+          oidx[b] = -1;
+          oval[b] = cf{0.f, 0.f};
+        }
+      }
+    }
+    // the other buffer was last read before this pair's first barrier
+    for (int b = t; b < nb; b += TPT) bins_next[b] = 0ull;
+  }
+}
+
+}  // namespace mfh

@@ -1,0 +1,178 @@
+// Host launcher for the fused flat kernel: per-length instantiation, LDS
+// sizing, window splitting and the data-tile choice.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "mfh_flat.h"
+#include "mfh_internal.h"
+
+namespace mfh {
+
+namespace {
+
+// LDS a workgroup may claim on gfx950 (160 KiB per CU).
+constexpr size_t kLdsLimit = 160 * 1024;
+
+template <int N>
+size_t flat_lds_bytes(size_t nb) {
+  using S = FlatShape<N>;
+  return (size_t)S::SLOTS * S::XCH * sizeof(cf) + 2 * (size_t)S::SLOTS * nb * sizeof(uint64_t);
+}
+
+template <int N>
+size_t flat_max_bins() {
+  using S = FlatShape<N>;
+  const size_t fixed = (size_t)S::SLOTS * S::XCH * sizeof(cf);
+  return (kLdsLimit - fixed) / (2 * (size_t)S::SLOTS * sizeof(uint64_t));
+}
+
+template <int N>
+int launch_n(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, size_t bin0, size_t nb) {
+  using S = FlatShape<N>;
+  FlatArgs a;
+  a.data = r.data;
+  a.tmpl = r.tmpl;
+  a.twiddle = c->tw;
+  a.data_stride = r.data_stride;
+  a.tmpl_stride = r.tmpl_stride;
+  a.nd = r.nd;
+  a.nt = r.nt;
+  a.ws = (uint32_t)ws;
+  a.we = (uint32_t)we;
+  a.binsize = (uint32_t)r.binsize;
+  a.nb = (uint32_t)nb;
+  a.ostride = (uint32_t)nb_total;
+  const bool pow2 = (r.binsize & (r.binsize - 1)) == 0;
+  a.bshift = pow2 ? ilog2((int)r.binsize) : -1;
+  a.inv_bs = 1.0f / (float)r.binsize;
+  const float thr = r.threshold;
+  a.seed = thr > 0.f ? (float)((double)thr * (double)thr) : 0.f;
+  // Output rows are [nd][nt][nb_total]; a window piece writes bins
+  // [bin0, bin0 + nb) of each row.
+  a.out_idx = r.out_idx + bin0;
+  a.out_val = r.out_val + bin0;
+  int dpw = c->tune_dpw;
+  if (dpw <= 0) dpw = std::min(r.nd, 16);
+  a.dpw = dpw;
+  dim3 grid((unsigned)((r.nt + S::SLOTS - 1) / S::SLOTS), (unsigned)((r.nd + dpw - 1) / dpw));
+  const size_t lds = flat_lds_bytes<N>(nb);
+  static size_t lds_set = 0;
+  if (lds > lds_set) {
+    if (!hip_ok(hipFuncSetAttribute((const void *)flat_kernel<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds),
+                "hipFuncSetAttribute"))
+      return -1;
+    lds_set = lds;
+  }
+  if (grid.y > 65535) return fail("too many data rows per launch (%d)", r.nd);
+  Ctx::ProfScope ps(c, "flat");
+  hipLaunchKernelGGL(flat_kernel<N>, grid, dim3(S::WG), lds, c->stream, a);
+  return hip_ok(hipGetLastError(), "flat_kernel launch") ? 0 : -1;
+}
+
+template <int N>
+int launch_len(Ctx *c, const FlatRun &r) {
+  const size_t nb = (r.we - r.ws + r.binsize - 1) / r.binsize;
+  const size_t maxb = flat_max_bins<N>();
+  if (nb <= maxb) return launch_n<N>(c, r, r.ws, r.we, nb, 0, nb);
+  // Too many bins for one LDS bin table: split the window into pieces of
+  // whole bins.  Each piece recomputes the transform; bins are independent.
+  for (size_t b0 = 0; b0 < nb; b0 += maxb) {
+    const size_t cnt = std::min(maxb, nb - b0);
+    const size_t ws = r.ws + b0 * r.binsize;
+    const size_t we = std::min(r.we, ws + cnt * r.binsize);
+    if (launch_n<N>(c, r, ws, we, nb, b0, cnt) < 0) return -1;
+  }
+  return 0;
+}
+
+// exp(+2 pi i r k / (R ns)) at [k][r], k < ns, r < R, generated in FP64.
+__global__ void twiddle_pass_kernel(cf *out, int ns, int R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ns * R) {
+    const int k = i / R, r = i % R;
+    double s, c;
+    sincospi(2.0 * (double)r * (double)k / (double)(R * ns), &s, &c);
+    out[i] = cf{(float)c, (float)s};
+  }
+}
+
+template <int N>
+int fill_twiddles(Ctx *c) {
+  using L = TwiddleLayout<N>;
+  using S = FlatShape<N>;
+  int ns = L::NS0;
+  for (int p = 0; p < S::NPASS16; p++, ns *= 16) {
+    if (ns == 1) continue;
+    const int cnt = ns * 16;
+    hipLaunchKernelGGL(twiddle_pass_kernel, dim3((cnt + 255) / 256), dim3(256), 0, c->stream,
+                       c->tw + L::offset(p), ns, 16);
+  }
+  return hip_ok(hipGetLastError(), "twiddle launch") ? 0 : -1;
+}
+
+}  // namespace
+
+size_t flat_twiddle_count(size_t n) {
+  switch (n) {
+    case 64: return TwiddleLayout<64>::total();
+    case 128: return TwiddleLayout<128>::total();
+    case 256: return TwiddleLayout<256>::total();
+    case 512: return TwiddleLayout<512>::total();
+    case 1024: return TwiddleLayout<1024>::total();
+    case 2048: return TwiddleLayout<2048>::total();
+    case 4096: return TwiddleLayout<4096>::total();
+    case 8192: return TwiddleLayout<8192>::total();
+    case 16384: return TwiddleLayout<16384>::total();
+    default: return 0;
+  }
+}
+
+int flat_fill_twiddles(Ctx *c) {
+  switch (c->n) {
+    case 64: return fill_twiddles<64>(c);
+    case 128: return fill_twiddles<128>(c);
+    case 256: return fill_twiddles<256>(c);
+    case 512: return fill_twiddles<512>(c);
+    case 1024: return fill_twiddles<1024>(c);
+    case 2048: return fill_twiddles<2048>(c);
+    case 4096: return fill_twiddles<4096>(c);
+    case 8192: return fill_twiddles<8192>(c);
+    case 16384: return fill_twiddles<16384>(c);
+    default: return fail("transform length %zu is not supported", c->n);
+  }
+}
+
+bool flat_supported(size_t n) {
+  switch (n) {
+    case 64: case 128: case 256: case 512: case 1024: case 2048: case 4096: case 8192: case 16384:
+      return true;
+    default:
+      return false;
+  }
+}
+
+int launch_flat(Ctx *c, const FlatRun &r) {
+  if (r.binsize < 1) return fail("binsize must be >= 1");
+  size_t we = std::min(r.we, c->n);
+  if (r.ws >= we) return fail("empty window (%zu, %zu)", r.ws, r.we);
+  FlatRun q = r;
+  q.we = we;
+  hipSetDevice(c->dev);
+  switch (c->n) {
+    case 64: return launch_len<64>(c, q);
+    case 128: return launch_len<128>(c, q);
+    case 256: return launch_len<256>(c, q);
+    case 512: return launch_len<512>(c, q);
+    case 1024: return launch_len<1024>(c, q);
+    case 2048: return launch_len<2048>(c, q);
+    case 4096: return launch_len<4096>(c, q);
+    case 8192: return launch_len<8192>(c, q);
+    case 16384: return launch_len<16384>(c, q);
+    default: return fail("transform length %zu is not supported", c->n);
+  }
+}
+
+}  // namespace mfh

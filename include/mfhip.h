@@ -1,0 +1,122 @@
+/* mfhip -- MI355X (gfx950) batched matched filter, C ABI.
+ *
+ * The library behind apogee_amd's device="gpu" path.  Every entry point is
+ * extern "C", takes plain pointers and sizes, returns an int status (< 0 is an
+ * error whose text mfh_last_error() returns for the calling thread), and is
+ * synchronous on return unless its name ends in _async, in which case the work
+ * is queued on the context's HIP stream (mfh_stream) and mfh_sync() waits.
+ *
+ * Spectra are complex64 (interleaved float re, im), natural frequency order,
+ * the unnormalised forward transform of each segment.  For a (data d,
+ * template t) pair the filter evaluates
+ *     z[k] = sum_f D_d[f] conj(T_t[f]) exp(+2 pi i f k / n)
+ * and reports, per bin of the lag window, the lag of the largest |z| and z
+ * there; a bin whose best |z|^2 does not exceed threshold^2 reports index -1
+ * and value 0+0j.  Among equal |z|^2 the lowest lag is reported.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the reference checkout).  The reference's GPU contract is the Python
+ * Context protocol of python/matchedfilter/_vkcompute.py; its CPU contract is
+ * python/matchedfilter/matchedfilter.h.
+ */
+#ifndef MFHIP_H
+#define MFHIP_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mfh_ctx mfh_ctx;
+
+/* Last error message of the calling thread ("" if none).
+ * Replaces: VulkanError text (python/matchedfilter/_vkcompute.py:252-258). */
+const char *mfh_last_error(void);
+
+/* Number of HIP devices visible to the process (0 when none).
+ * Replaces: GPU enumeration (python/matchedfilter/_vulkan.py:enumerate_devices,
+ * python/matchedfilter/device.py:114-142). */
+int mfh_device_count(void);
+
+/* Name and gfx architecture of device `index` into caller buffers.
+ * Replaces: Device.name / arch_keys (python/matchedfilter/device.py:20-56). */
+int mfh_device_info(int index, char *name, size_t name_len, char *arch, size_t arch_len);
+
+/* 1 if the transform length n is supported by this build, else 0.
+ * Replaces: _GPU_SIZES (python/matchedfilter/__init__.py:58-59) and
+ * ap_supported (python/matchedfilter/matchedfilter.h:204). */
+int mfh_supported(size_t n);
+
+/* Create a filter context on `device` for length n with room for nd data and
+ * nt template spectra.  NULL on failure.
+ * Replaces: Context(index) (python/matchedfilter/_vkcompute.py:962 class) and
+ * ap_mf_create (python/matchedfilter/matchedfilter.h:53). */
+mfh_ctx *mfh_create(int device, size_t n, int nd, int nt);
+
+/* Release a context and its device memory.
+ * Replaces: Context.destroy (python/matchedfilter/_vkcompute.py) and
+ * ap_mf_destroy (matchedfilter.h:54). */
+void mfh_destroy(mfh_ctx *ctx);
+
+/* The context's HIP stream (hipStream_t), for event timing by the caller. */
+void *mfh_stream(mfh_ctx *ctx);
+
+/* Wait for all work queued on the context's stream. */
+int mfh_sync(mfh_ctx *ctx);
+
+/* Copy `rows` spectra (rows x n complex64, host memory) into data rows
+ * [row0, row0 + rows).  The library copies; the caller's buffer is free on
+ * return.
+ * Replaces: ap_mf_set_data (matchedfilter.h:60) and the upload of
+ * Context.peaks(upload_data=True) (python/matchedfilter/_vkcompute.py:1042-1047). */
+int mfh_set_data(mfh_ctx *ctx, int row0, int rows, const void *host_c64);
+
+/* Same for template rows.  No conjugation is stored: the kernels conjugate.
+ * Replaces: ap_mf_set_template (matchedfilter.h:61). */
+int mfh_set_templates(mfh_ctx *ctx, int row0, int rows, const void *host_c64);
+
+/* Device-resident variants: `dev_c64` is a device pointer on the context's
+ * device (e.g. a torch tensor); copied device to device, queued on the stream.
+ * Replaces: nothing in the reference (it refuses device arrays,
+ * python/matchedfilter/__init__.py:103-137); listed as the zero-copy ingest
+ * follow-up of the hot path. */
+int mfh_set_data_device_async(mfh_ctx *ctx, int row0, int rows, const void *dev_c64);
+int mfh_set_templates_device_async(mfh_ctx *ctx, int row0, int rows, const void *dev_c64);
+
+/* Number of bins of window [ws, we) at `binsize` (0 if empty).
+ * Replaces: ap_mf_nbins (matchedfilter.h:56; src/matchfilt.c:151-156). */
+size_t mfh_nbins(size_t n, size_t binsize, size_t ws, size_t we);
+
+/* Flat filter over data rows [d0, d0+nd) x template rows [t0, t0+nt), lags
+ * [ws, we) in bins of `binsize`.  Writes host arrays idx (int64) and val
+ * (complex64) of shape [nd][nt][nbins]; if counts is not NULL it receives the
+ * per-pair number of bins with index >= 0.  Returns the total number of such
+ * bins, or < 0 on error.
+ * Replaces: Context.peaks (python/matchedfilter/_vkcompute.py:962-1063) and
+ * ap_mf_run (matchedfilter.h:66-68). */
+long mfh_flat_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
+                    size_t binsize, float threshold, size_t ws, size_t we,
+                    int64_t *idx, void *val_c64, int32_t *counts);
+
+/* Same with device output pointers; queued on the context stream, no sync.
+ * Returns 0 or < 0.  Replaces: the fusedTierB dispatch recorded by
+ * Context._make_batch (python/matchedfilter/_vkcompute.py:923-960). */
+int mfh_flat_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
+                         size_t binsize, float threshold, size_t ws, size_t we,
+                         int64_t *dev_idx, void *dev_val_c64);
+
+/* Tuning knobs: key "dpw" (data rows per workgroup; 0 = automatic).
+ * Replaces: MF_MFTILE / MF_PBMAX environment knobs (src/matchfilt.c:93-107). */
+int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value);
+
+/* Per-kernel timing with HIP events on the context stream: enable (1) or
+ * disable (0) recording, and read the accumulated milliseconds and launch
+ * count of kernel `name` ("flat", ...).  Enabling clears the totals.
+ * Replaces: the MF_HMF_PROF stage timers (src/hmf.c:65, :94-101). */
+int mfh_prof_enable(mfh_ctx *ctx, int on);
+int mfh_prof_read(mfh_ctx *ctx, const char *name, double *total_ms, long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -57,6 +57,28 @@ def _declare(lib):
     return lib
 
 
+def _share_torch_hip_runtime():
+    """Make libmfhip.so and PyTorch use ONE HIP runtime.
+
+    PyTorch-ROCm bundles its own libamdhip64 and names it in DT_NEEDED as
+    "libamdhip64.so"; libmfhip.so names the system runtime by its soname
+    "libamdhip64.so.7".  Loaded in the other order the process ends up with
+    two runtimes and PyTorch's finds no device.  Preloading PyTorch's copy
+    (same soname) first makes both resolve to it.  Nothing happens when
+    PyTorch is not installed.
+    """
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        spec = None
+    if spec is None or not spec.submodule_search_locations:
+        return
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
 def lib():
     """The loaded HIP library; raises RuntimeError if it was never built."""
     global _lib
@@ -66,6 +88,7 @@ def lib():
                 raise RuntimeError(
                     "the HIP library %s is missing: build it with "
                     "`python -m apogee_amd._build` (hipcc, gfx950)" % LIB_PATH)
+            _share_torch_hip_runtime()
             _lib = _declare(ctypes.CDLL(LIB_PATH))
         return _lib
 
@@ -160,6 +183,9 @@ class Context:
         rows = int(spectra.shape[0])
         L = lib()
         if dev is not None:
+            # the producer's work runs on PyTorch's stream: finish it first
+            import torch
+            torch.cuda.current_stream(spectra.device).synchronize()
             fn = L.mfh_set_data_device_async if kind == "data" else L.mfh_set_templates_device_async
             check(fn(self.handle, int(row0), rows, ctypes.c_void_p(dev)), "set %s" % kind)
             self.sync()

@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/mfhip.h"
+#include "mfhip.h"
 #include "mfh_flat.h"
 #include "mfh_internal.h"
 

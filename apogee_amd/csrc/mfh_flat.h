@@ -83,10 +83,13 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
   return v;
 }
 
-// Twiddle tables, one per twiddled pass, laid out [k][r] so that a thread's R
-// twiddles sit at one base address plus compile-time offsets:
-//   tw[pass_offset(NS) + k*R + r] = exp(+2 pi i r k / (R NS)),  k < NS, r < R.
-// Passes with NS = 1 need none.  Generated in FP64 on the device.
+// Twiddle tables, one per twiddled radix-16 pass, holding only the powers
+// r = 1, 2, 4, 8 of each butterfly's twiddle, laid out [row][k]:
+//   tw[pass_offset(p) + row*NS + k] = exp(+2 pi i (1 << row) k / (16 NS)),
+// k < NS, row < 4.  For each row consecutive lanes read consecutive
+// elements, so each of the 4 loads is one contiguous run; the other 11
+// powers are products of these (at most 3 roundings deep, ~1e-7 relative),
+// which costs 44 FMA-class instructions instead of 11 more gathers.
 template <int N>
 struct TwiddleLayout {
   using S = FlatShape<N>;
@@ -95,7 +98,7 @@ struct TwiddleLayout {
   static constexpr int offset(int p) {
     int off = 0, ns = NS0;
     for (int i = 0; i < p; i++) {
-      if (ns > 1) off += ns * 16;
+      if (ns > 1) off += ns * 4;
       ns *= 16;
     }
     return off;
@@ -103,23 +106,41 @@ struct TwiddleLayout {
   static constexpr int total() { return offset(S::NPASS16); }
 };
 
+// Twiddles of one radix-16 pass for butterfly j = t (one butterfly per
+// thread): w[r-1] = exp(+2 pi i r (t mod NS) / (16 NS)), r = 1..15.  Issued
+// ahead of the barrier that precedes the pass, so their latency hides there.
+template <int N, int NS>
+__device__ __forceinline__ void load_twiddles16(cf *w, int t, const cf *tw) {
+  const cf *p = tw + (t % NS);
+  const cf w1 = p[0], w2 = p[NS], w4 = p[2 * NS], w8 = p[3 * NS];
+  w[0] = w1;
+  w[1] = w2;
+  w[3] = w4;
+  w[7] = w8;
+  w[2] = cmul(w1, w2);
+  w[4] = cmul(w1, w4);
+  w[5] = cmul(w2, w4);
+  w[6] = cmul(w[2], w4);
+#pragma unroll
+  for (int r = 9; r < 16; r++) w[r - 1] = cmul(w[r - 9], w8);
+}
+
 // One Stockham pass of radix R: butterflies j = t + q*TPT (q < 16/R) read
 // x[j + r*N/R] = registers v[q + r*Q], twiddle by exp(+2 pi i r (j mod NS) /
-// (R NS)) and run the R-point DFT in place.
+// (R NS)) and run the R-point DFT in place.  `w` holds the radix-16
+// twiddles of this thread's butterfly when NS > 1 (then R = 16, Q = 1).
 template <int N, int R, int NS>
-__device__ __forceinline__ void stockham_pass(cf *v, int t, const cf *tw) {
-  constexpr int TPT = N / 16;
+__device__ __forceinline__ void stockham_pass(cf *v, const cf *w) {
   constexpr int Q = 16 / R;  // butterflies per thread
+  static_assert(NS == 1 || R == 16, "twiddled passes are radix 16");
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     cf b[R];
 #pragma unroll
     for (int r = 0; r < R; r++) b[r] = v[q + r * Q];
     if constexpr (NS > 1) {
-      const int k = (t + q * TPT) % NS;
-      const cf *w = tw + k * R;
 #pragma unroll
-      for (int r = 1; r < R; r++) b[r] = cmul(b[r], w[r]);
+      for (int r = 1; r < R; r++) b[r] = cmul(b[r], w[r - 1]);
     }
     dftR<R>(b);
 #pragma unroll
@@ -168,39 +189,52 @@ __device__ __forceinline__ void stockham_load(cf *v, int t, const cf *xch) {
 }
 
 // Full inverse FFT of the slot's N points held in v; returns with thread t
-// holding z[t + u*TPT].  Two barriers per exchange.
+// holding z[t + u*TPT].  Each exchange is store -> barrier -> load, and the
+// barrier that protects the buffer from the next store sits just before that
+// store, after the pass's arithmetic, so waves do not idle between load and
+// compute.  The caller must barrier before the next call's first store.
 template <int N>
 __device__ __forceinline__ void inverse_fft(cf *v, int t, cf *xch, const cf *tw) {
   using S = FlatShape<N>;
   using L = TwiddleLayout<N>;
   constexpr int F = S::FIRST;
-  if constexpr (F > 1) {
-    stockham_pass<N, F, 1>(v, t, tw);
-    stockham_store<N, F, 1>(v, t, xch);
-    __syncthreads();
-    stockham_load<N>(v, t, xch);
-    __syncthreads();
-  }
   constexpr int NS0 = L::NS0;
-  stockham_pass<N, 16, NS0>(v, t, tw + L::offset(0));
-  if constexpr (S::NPASS16 >= 2) {
-    stockham_store<N, 16, NS0>(v, t, xch);
+  cf w[15];
+  if constexpr (F > 1) {
+    stockham_pass<N, F, 1>(v, w);
+    stockham_store<N, F, 1>(v, t, xch);
+    load_twiddles16<N, NS0>(w, t, tw + L::offset(0));
     __syncthreads();
     stockham_load<N>(v, t, xch);
+  }
+  stockham_pass<N, 16, NS0>(v, w);
+  if constexpr (S::NPASS16 >= 2) {
+    if constexpr (F > 1) __syncthreads();
+    stockham_store<N, 16, NS0>(v, t, xch);
+    load_twiddles16<N, NS0 * 16>(w, t, tw + L::offset(1));
     __syncthreads();
-    stockham_pass<N, 16, NS0 * 16>(v, t, tw + L::offset(1));
+    stockham_load<N>(v, t, xch);
+    stockham_pass<N, 16, NS0 * 16>(v, w);
   }
   if constexpr (S::NPASS16 >= 3) {
+    __syncthreads();
     stockham_store<N, 16, NS0 * 16>(v, t, xch);
+    load_twiddles16<N, NS0 * 256>(w, t, tw + L::offset(2));
     __syncthreads();
     stockham_load<N>(v, t, xch);
-    __syncthreads();
-    stockham_pass<N, 16, NS0 * 256>(v, t, tw + L::offset(2));
+    stockham_pass<N, 16, NS0 * 256>(v, w);
   }
   static_assert(S::NPASS16 >= 1 && S::NPASS16 <= 3, "N range");
 }
 
-template <int N>
+// Bin-table modes of the flat kernel.
+enum BinMode : int {
+  kOneBin = 0,   // a single bin over the whole window
+  kAligned = 1,  // window and binsize multiples of 64: every bin is wave-uniform
+  kGeneral = 2,  // anything else
+};
+
+template <int N, int MODE>
 __global__ void __launch_bounds__(FlatShape<N>::WG) __attribute__((amdgpu_waves_per_eu(4)))
 flat_kernel(FlatArgs a) {
   using S = FlatShape<N>;
@@ -247,80 +281,125 @@ flat_kernel(FlatArgs a) {
     }
     inverse_fft<N>(v, t, xch, a.twiddle);
 
-    // ---- per-bin maximum of the keys ---------------------------------
-    if constexpr (TPT >= 64) {
-      uint64_t acc = 0ull;
-      int acc_bin = -1;
+    if constexpr (MODE == kOneBin) {
+      // ---- one bin: thread-local argmax, one wave reduction, one LDS max ----
+      float bm = a.seed;
+      int bu = -1;
+      cf bz = cf{0.f, 0.f};
 #pragma unroll
       for (int u = 0; u < 16; u++) {
         const uint32_t k = (uint32_t)(t + u * TPT);
-        const uint32_t kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
-        const uint64_t key = (k >= a.ws && k < a.we) ? make_key(v[u], k, a.seed) : 0ull;
-        const bool uni = kf >= a.ws && kf + 63 < a.we && bin_of(kf, a) == bin_of(kf + 63, a);
-        if (uni) {
-          const int b = (int)bin_of(kf, a);
-          if (b != acc_bin) {
-            if (acc_bin >= 0) {
-              const uint64_t w = wave_max_u64(acc);
-              if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
-            }
-            acc = 0ull;
-            acc_bin = b;
-          }
-          acc = key > acc ? key : acc;
-        } else if (key) {
-          atomicMax((unsigned long long *)&bins[bin_of(k, a)], (unsigned long long)key);
+        const float m2 = __builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y);
+        if (k >= a.ws && k < a.we && m2 > bm) {  // strict: the lowest lag keeps a tie
+          bm = m2;
+          bu = u;
+          bz = v[u];
         }
       }
-      if (acc_bin >= 0) {
-        const uint64_t w = wave_max_u64(acc);
-        if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
+      const uint32_t kb = (uint32_t)(t + bu * TPT);
+      const uint64_t key = bu >= 0 ? ((uint64_t)__float_as_uint(bm) << 32) | (uint64_t)(~kb) : 0ull;
+      if constexpr (TPT >= 64) {
+        const uint64_t w = wave_max_u64(key);
+        if (lane == 0 && w) atomicMax((unsigned long long *)&bins[0], (unsigned long long)w);
+      } else {
+        if (key) atomicMax((unsigned long long *)&bins[0], (unsigned long long)key);
+      }
+      __syncthreads();
+      if (tvalid) {
+        const long long row = (long long)d * a.nt + tt;
+        const uint64_t win = bins[0];
+        if (key && key == win) {
+          a.out_idx[row * a.ostride] = (int64_t)kb;
+          a.out_val[row * a.ostride] = bz;
+        }
+        if (t == 0 && win == 0ull) {
+          a.out_idx[row * a.ostride] = -1;
+          a.out_val[row * a.ostride] = cf{0.f, 0.f};
+        }
       }
     } else {
-      // several transforms share a wave: per-thread runs, then LDS atomics
-      uint64_t acc = 0ull;
-      int acc_bin = -1;
+      // ---- several bins: per-thread runs, wave reductions, LDS max ----------
+      if constexpr (TPT >= 64) {
+        uint64_t acc = 0ull;
+        int acc_bin = -1;
 #pragma unroll
-      for (int u = 0; u < 16; u++) {
-        const uint32_t k = (uint32_t)(t + u * TPT);
-        if (k >= a.ws && k < a.we) {
-          const uint64_t key = make_key(v[u], k, a.seed);
-          const int b = (int)bin_of(k, a);
-          if (b != acc_bin) {
-            if (acc) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)acc);
-            acc = 0ull;
-            acc_bin = b;
+        for (int u = 0; u < 16; u++) {
+          const uint32_t k = (uint32_t)(t + u * TPT);
+          const uint64_t key = (k >= a.ws && k < a.we) ? make_key(v[u], k, a.seed) : 0ull;
+          bool uni;
+          uint32_t kf = 0;
+          if constexpr (MODE == kAligned) {
+            kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
+            uni = kf >= a.ws && kf < a.we;
+          } else {
+            kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
+            uni = kf >= a.ws && kf + 63 < a.we && bin_of(kf, a) == bin_of(kf + 63, a);
           }
-          acc = key > acc ? key : acc;
+          if (uni) {
+            const int b = (int)bin_of(kf, a);
+            if (b != acc_bin) {
+              if (acc_bin >= 0) {
+                const uint64_t w = wave_max_u64(acc);
+                if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
+              }
+              acc = 0ull;
+              acc_bin = b;
+            }
+            acc = key > acc ? key : acc;
+          } else if (key) {
+            atomicMax((unsigned long long *)&bins[bin_of(k, a)], (unsigned long long)key);
+          }
         }
-      }
-      if (acc) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)acc);
-    }
-    __syncthreads();
-
-    // ---- the owner of each winning lag writes it; empty bins get -1/0 ----
-    if (tvalid) {
-      const long long row = (long long)d * a.nt + tt;
-      int64_t *oidx = a.out_idx + row * a.ostride;
-      cf *oval = a.out_val + row * a.ostride;
+        if (acc_bin >= 0) {
+          const uint64_t w = wave_max_u64(acc);
+          if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
+        }
+      } else {
+        // several transforms share a wave: per-thread runs, then LDS atomics
+        uint64_t acc = 0ull;
+        int acc_bin = -1;
 #pragma unroll
-      for (int u = 0; u < 16; u++) {
-        const uint32_t k = (uint32_t)(t + u * TPT);
-        if (k >= a.ws && k < a.we) {
-          const uint64_t key = make_key(v[u], k, a.seed);
-          if (key) {
-            const uint32_t b = bin_of(k, a);
-            if (bins[b] == key) {
-              oidx[b] = (int64_t)k;
-              oval[b] = v[u];
+        for (int u = 0; u < 16; u++) {
+          const uint32_t k = (uint32_t)(t + u * TPT);
+          if (k >= a.ws && k < a.we) {
+            const uint64_t key = make_key(v[u], k, a.seed);
+            const int b = (int)bin_of(k, a);
+            if (b != acc_bin) {
+              if (acc) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)acc);
+              acc = 0ull;
+              acc_bin = b;
+            }
+            acc = key > acc ? key : acc;
+          }
+        }
+        if (acc) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)acc);
+      }
+      __syncthreads();
+
+      // ---- the owner of each winning lag writes it; empty bins get -1/0 ----
+      if (tvalid) {
+        const long long row = (long long)d * a.nt + tt;
+        int64_t *oidx = a.out_idx + row * a.ostride;
+        cf *oval = a.out_val + row * a.ostride;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const uint32_t k = (uint32_t)(t + u * TPT);
+          if (k >= a.ws && k < a.we) {
+            const uint64_t key = make_key(v[u], k, a.seed);
+            if (key) {
+              const uint32_t b = bin_of(k, a);
+              if (bins[b] == key) {
+                oidx[b] = (int64_t)k;
+                oval[b] = v[u];
+              }
             }
           }
         }
-      }
-      for (int b = t; b < nb; b += TPT) {
-        if (bins[b] == 0ull) {
-          oidx[b] = -1;
-          oval[b] = cf{0.f, 0.f};
+        for (int b = t; b < nb; b += TPT) {
+          if (bins[b] == 0ull) {
+            oidx[b] = -1;
+            oval[b] = cf{0.f, 0.f};
+          }
         }
       }
     }

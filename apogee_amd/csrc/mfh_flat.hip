@@ -58,17 +58,27 @@ int launch_n(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, si
   a.dpw = dpw;
   dim3 grid((unsigned)((r.nt + S::SLOTS - 1) / S::SLOTS), (unsigned)((r.nd + dpw - 1) / dpw));
   const size_t lds = flat_lds_bytes<N>(nb);
-  static size_t lds_set = 0;
-  if (lds > lds_set) {
-    if (!hip_ok(hipFuncSetAttribute((const void *)flat_kernel<N>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds),
+  int mode = kGeneral;
+  if (nb == 1) mode = kOneBin;
+  else if (ws % 64 == 0 && (we % 64 == 0) && r.binsize % 64 == 0) mode = kAligned;
+  const void *fn = mode == kOneBin ? (const void *)flat_kernel<N, kOneBin>
+                 : mode == kAligned ? (const void *)flat_kernel<N, kAligned>
+                                    : (const void *)flat_kernel<N, kGeneral>;
+  static size_t lds_set[3] = {0, 0, 0};
+  if (lds > lds_set[mode]) {
+    if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                 "hipFuncSetAttribute"))
       return -1;
-    lds_set = lds;
+    lds_set[mode] = lds;
   }
   if (grid.y > 65535) return fail("too many data rows per launch (%d)", r.nd);
   Ctx::ProfScope ps(c, "flat");
-  hipLaunchKernelGGL(flat_kernel<N>, grid, dim3(S::WG), lds, c->stream, a);
+  if (mode == kOneBin)
+    hipLaunchKernelGGL((flat_kernel<N, kOneBin>), grid, dim3(S::WG), lds, c->stream, a);
+  else if (mode == kAligned)
+    hipLaunchKernelGGL((flat_kernel<N, kAligned>), grid, dim3(S::WG), lds, c->stream, a);
+  else
+    hipLaunchKernelGGL((flat_kernel<N, kGeneral>), grid, dim3(S::WG), lds, c->stream, a);
   return hip_ok(hipGetLastError(), "flat_kernel launch") ? 0 : -1;
 }
 
@@ -88,13 +98,14 @@ int launch_len(Ctx *c, const FlatRun &r) {
   return 0;
 }
 
-// exp(+2 pi i r k / (R ns)) at [k][r], k < ns, r < R, generated in FP64.
-__global__ void twiddle_pass_kernel(cf *out, int ns, int R) {
+// Rows r = 1, 2, 4, 8 of exp(+2 pi i r k / (16 ns)), k < ns, at [row][k],
+// generated in FP64.
+__global__ void twiddle_pass_kernel(cf *out, int ns) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ns * R) {
-    const int k = i / R, r = i % R;
+  if (i < ns * 4) {
+    const int r = 1 << (i / ns), k = i % ns;
     double s, c;
-    sincospi(2.0 * (double)r * (double)k / (double)(R * ns), &s, &c);
+    sincospi(2.0 * (double)r * (double)k / (double)(16 * ns), &s, &c);
     out[i] = cf{(float)c, (float)s};
   }
 }
@@ -106,9 +117,9 @@ int fill_twiddles(Ctx *c) {
   int ns = L::NS0;
   for (int p = 0; p < S::NPASS16; p++, ns *= 16) {
     if (ns == 1) continue;
-    const int cnt = ns * 16;
+    const int cnt = ns * 4;
     hipLaunchKernelGGL(twiddle_pass_kernel, dim3((cnt + 255) / 256), dim3(256), 0, c->stream,
-                       c->tw + L::offset(p), ns, 16);
+                       c->tw + L::offset(p), ns);
   }
   return hip_ok(hipGetLastError(), "twiddle launch") ? 0 : -1;
 }

@@ -87,7 +87,20 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain 256 MiB device copy (read + write) five times, "
+                         "to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": src.numel() * 4,
+                          "copy_probe_bytes_written": dst.numel() * 4, "launches": 5}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the matched filter shards by template bank "
               "with no collective in the hot loop, and this build runs on one device "

@@ -50,6 +50,14 @@ def _declare(lib):
                                    _c_size, ctypes.c_float, _c_size, _c_size, _vp, _vp, _vp]
     lib.mfh_flat_peaks_async.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _c_size, ctypes.c_float, _c_size, _c_size, _vp, _vp]
+    lib.mfh_hier_set_band.argtypes = [_vp, _c_size, ctypes.c_double]
+    lib.mfh_hier_peaks.restype = ctypes.c_long
+    lib.mfh_hier_peaks.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   _c_size, ctypes.c_float, _c_size, _c_size, ctypes.c_float,
+                                   _vp, _vp, _vp, _vp]
+    lib.mfh_hier_peaks_async.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         _c_size, ctypes.c_float, _c_size, _c_size, ctypes.c_float,
+                                         _vp, _vp, _vp]
     lib.mfh_set_tuning.argtypes = [_vp, ctypes.c_char_p, ctypes.c_long]
     lib.mfh_prof_enable.argtypes = [_vp, ctypes.c_int]
     lib.mfh_prof_read.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
@@ -215,6 +223,33 @@ class Context:
                                          float(threshold), int(start), int(end),
                                          ctypes.c_void_p(int(idx_ptr)), ctypes.c_void_p(int(val_ptr))),
               "mfh_flat_peaks_async")
+
+    # ---------------------------------------------------------- hierarchy
+    def hier_set_band(self, band, ref_fraction=-1.0):
+        """Coarse band and normalisation (bank-wide fraction, or < 0 per template)."""
+        check(lib().mfh_hier_set_band(self.handle, int(band), float(ref_fraction)), "mfh_hier_set_band")
+
+    def hier_peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, coarse_threshold, counts=False):
+        """Hierarchical run; returns (index, value, counts or None, refined pairs)."""
+        nb = int(lib().mfh_nbins(self.n, int(binsize), int(start), int(end)))
+        idx = np.empty((nd, nt, nb), dtype=np.int64)
+        val = np.empty((nd, nt, nb), dtype=np.complex64)
+        cnt = np.empty((nd, nt), dtype=np.int32) if counts else None
+        refined = ctypes.c_int64(0)
+        check(lib().mfh_hier_peaks(self.handle, int(d0), int(nd), int(t0), int(nt), int(binsize),
+                                   float(threshold), int(start), int(end), float(coarse_threshold),
+                                   _ptr(idx), _ptr(val), _ptr(cnt) if cnt is not None else None,
+                                   ctypes.byref(refined)), "mfh_hier_peaks")
+        self.last_refinements = int(refined.value)
+        return idx, val, cnt, int(refined.value)
+
+    def hier_peaks_async(self, d0, nd, t0, nt, binsize, threshold, start, end, coarse_threshold,
+                         idx_ptr, val_ptr):
+        """Queue a hierarchical run writing device buffers."""
+        check(lib().mfh_hier_peaks_async(self.handle, int(d0), int(nd), int(t0), int(nt), int(binsize),
+                                         float(threshold), int(start), int(end), float(coarse_threshold),
+                                         ctypes.c_void_p(int(idx_ptr)), ctypes.c_void_p(int(val_ptr)), None),
+              "mfh_hier_peaks_async")
 
     # ------------------------------------------------------------- tuning
     def set_tuning(self, key, value):

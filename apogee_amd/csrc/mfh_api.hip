@@ -11,7 +11,6 @@
 #include <vector>
 
 #include "mfhip.h"
-#include "mfh_flat.h"
 #include "mfh_internal.h"
 
 using mfh::cf;
@@ -46,6 +45,17 @@ Ctx::~Ctx() {
     hipEventDestroy(e.b);
   }
   if (stream) hipStreamDestroy(stream);
+}
+
+const cf *Ctx::twiddles(size_t len) {
+  auto it = tw_by_n.find(len);
+  if (it != tw_by_n.end()) return it->second;
+  char name[32];
+  snprintf(name, sizeof name, "twiddle%zu", len);
+  cf *tw = (cf *)scratch(name, (flat_twiddle_count(len) + 1) * sizeof(cf));
+  if (!tw || flat_fill_twiddles(this, len, tw) < 0) return nullptr;
+  tw_by_n[len] = tw;
+  return tw;
 }
 
 void *Ctx::scratch(const char *name, size_t bytes) {
@@ -146,15 +156,14 @@ extern "C" mfh_ctx *mfh_create(int device, size_t n, int nd, int nt) {
   }
   c->data = (cf *)c->scratch("data", (size_t)nd * n * sizeof(cf));
   c->tmpl = (cf *)c->scratch("tmpl", (size_t)nt * n * sizeof(cf));
-  c->tw = (cf *)c->scratch("twiddle", (mfh::flat_twiddle_count(n) + 1) * sizeof(cf));
-  if (!c->data || !c->tmpl || !c->tw) {
+  if (!c->data || !c->tmpl || !c->twiddles(n)) {
     delete c;
     return nullptr;
   }
   // unset rows read as zero rather than as whatever the allocator left
   hipMemsetAsync(c->data, 0, (size_t)nd * n * sizeof(cf), c->stream);
   hipMemsetAsync(c->tmpl, 0, (size_t)nt * n * sizeof(cf), c->stream);
-  if (mfh::flat_fill_twiddles(c) < 0 || !hip_ok(hipStreamSynchronize(c->stream), "create")) {
+  if (!hip_ok(hipStreamSynchronize(c->stream), "create")) {
     delete c;
     return nullptr;
   }
@@ -194,6 +203,7 @@ extern "C" int mfh_set_data(mfh_ctx *ctx, int row0, int rows, const void *host_c
 extern "C" int mfh_set_templates(mfh_ctx *ctx, int row0, int rows, const void *host_c64) {
   if (!ctx) return fail("null context");
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  c->coarse_dirty = true;
   return set_rows(c, c->tmpl, c->nt, row0, rows, host_c64, hipMemcpyHostToDevice, true);
 }
 extern "C" int mfh_set_data_device_async(mfh_ctx *ctx, int row0, int rows, const void *dev_c64) {
@@ -204,6 +214,7 @@ extern "C" int mfh_set_data_device_async(mfh_ctx *ctx, int row0, int rows, const
 extern "C" int mfh_set_templates_device_async(mfh_ctx *ctx, int row0, int rows, const void *dev_c64) {
   if (!ctx) return fail("null context");
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  c->coarse_dirty = true;
   return set_rows(c, c->tmpl, c->nt, row0, rows, dev_c64, hipMemcpyDeviceToDevice, false);
 }
 
@@ -233,8 +244,7 @@ extern "C" int mfh_flat_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt
   r.we = we;
   r.out_idx = dev_idx;
   r.out_val = (cf *)dev_val_c64;
-  r.out_row_stride = 0;
-  return mfh::launch_flat(c, r);
+  return mfh::launch_flat(c, c->n, r);
 }
 
 extern "C" long mfh_flat_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,

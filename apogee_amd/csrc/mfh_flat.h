@@ -37,7 +37,9 @@ struct FlatArgs {
   cf *out_val;         // [nd][nt][nb]
   long long data_stride, tmpl_stride;  // elements between rows
   int nd, nt;          // run extent
-  int dpw;             // data rows per workgroup
+  int dpw;             // data rows per workgroup (tiled mode)
+  const int2 *list;    // list mode: (data row, template row) per pair
+  const int *list_count;  // list mode: number of pairs in the list
   uint32_t ws, we;     // lag window [ws, we)
   uint32_t binsize, nb;  // nb: bins of this launch
   uint32_t ostride;      // output elements between pairs (>= nb)
@@ -234,7 +236,7 @@ enum BinMode : int {
   kGeneral = 2,  // anything else
 };
 
-template <int N, int MODE>
+template <int N, int MODE, bool LIST>
 __global__ void __launch_bounds__(FlatShape<N>::WG) __attribute__((amdgpu_waves_per_eu(4)))
 flat_kernel(FlatArgs a) {
   using S = FlatShape<N>;
@@ -249,23 +251,28 @@ flat_kernel(FlatArgs a) {
   cf *xch = xch_all + slot * S::XCH;
   const int nb = (int)a.nb;
 
-  const int tt = blockIdx.x * S::SLOTS + slot;
-  const bool tvalid = tt < a.nt;
-  const int dbeg = blockIdx.y * a.dpw;
-  const int dend = min(a.nd, dbeg + a.dpw);
-
-  // the template stays in registers for the whole data tile
+  // Tiled mode: this slot's template stays in registers while the workgroup
+  // walks a tile of data rows.  List mode: (data, template) pairs come from a
+  // device list whose length another kernel wrote, drained grid-stride.
+  int tt = blockIdx.x * S::SLOTS + slot;
+  bool valid = tt < a.nt;
+  int dbeg = blockIdx.y * a.dpw;
+  int dend = min(a.nd, dbeg + a.dpw);
   cf T[16];
-  {
-    const cf *tp = a.tmpl + (long long)(tvalid ? tt : 0) * a.tmpl_stride;
+  if constexpr (LIST) {
+    dbeg = blockIdx.x * S::SLOTS;
+    dend = *a.list_count;
+  } else {
+    const cf *tp = a.tmpl + (long long)(valid ? tt : 0) * a.tmpl_stride;
 #pragma unroll
-    for (int u = 0; u < 16; u++) T[u] = tvalid ? tp[t0 + u * TPT] : cf{0.f, 0.f};
+    for (int u = 0; u < 16; u++) T[u] = valid ? tp[t0 + u * TPT] : cf{0.f, 0.f};
   }
   // both bin buffers start empty
   for (int i = threadIdx.x; i < 2 * S::SLOTS * nb; i += S::WG) bins_all[i] = 0ull;
   __syncthreads();
 
-  for (int d = dbeg, it = 0; d < dend; d++, it++) {
+  const int dstep = LIST ? (int)gridDim.x * S::SLOTS : 1;
+  for (int dd = dbeg, it = 0; dd < dend; dd += dstep, it++) {
     // Everything derived from the thread index (bins, window flags, LDS and
     // twiddle addresses) is loop-invariant; recomputing it per pair is far
     // cheaper than the registers the hoisted copies would pin.
@@ -273,8 +280,19 @@ flat_kernel(FlatArgs a) {
     asm volatile("" : "+v"(t));
     uint64_t *bins = bins_all + ((it & 1) * S::SLOTS + slot) * nb;
     uint64_t *bins_next = bins_all + (((it + 1) & 1) * S::SLOTS + slot) * nb;
+    int d = dd;
     cf v[16];
-    {
+    if constexpr (LIST) {
+      const int li = dd + slot;
+      valid = li < dend;
+      const int2 pr = valid ? a.list[li] : int2{0, 0};
+      d = pr.x;
+      tt = pr.y;
+      const cf *dp = a.data + (long long)d * a.data_stride;
+      const cf *tp = a.tmpl + (long long)tt * a.tmpl_stride;
+#pragma unroll
+      for (int u = 0; u < 16; u++) v[u] = cmul_conj(dp[t + u * TPT], tp[t + u * TPT]);
+    } else {
       const cf *dp = a.data + (long long)d * a.data_stride;
 #pragma unroll
       for (int u = 0; u < 16; u++) v[u] = cmul_conj(dp[t + u * TPT], T[u]);
@@ -305,7 +323,7 @@ flat_kernel(FlatArgs a) {
         if (key) atomicMax((unsigned long long *)&bins[0], (unsigned long long)key);
       }
       __syncthreads();
-      if (tvalid) {
+      if (valid) {
         const long long row = (long long)d * a.nt + tt;
         const uint64_t win = bins[0];
         if (key && key == win) {
@@ -377,7 +395,7 @@ flat_kernel(FlatArgs a) {
       __syncthreads();
 
       // ---- the owner of each winning lag writes it; empty bins get -1/0 ----
-      if (tvalid) {
+      if (valid) {
         const long long row = (long long)d * a.nt + tt;
         int64_t *oidx = a.out_idx + row * a.ostride;
         cf *oval = a.out_val + row * a.ostride;

@@ -28,13 +28,29 @@ size_t flat_max_bins() {
   return (kLdsLimit - fixed) / (2 * (size_t)S::SLOTS * sizeof(uint64_t));
 }
 
+template <int N, int MODE, bool LIST>
+int launch_mode(Ctx *c, const FlatArgs &a, dim3 grid, size_t lds) {
+  using S = FlatShape<N>;
+  const void *fn = (const void *)flat_kernel<N, MODE, LIST>;
+  static size_t lds_set = 0;
+  if (lds > lds_set) {
+    if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                "hipFuncSetAttribute"))
+      return -1;
+    lds_set = lds;
+  }
+  hipLaunchKernelGGL((flat_kernel<N, MODE, LIST>), grid, dim3(S::WG), lds, c->stream, a);
+  return hip_ok(hipGetLastError(), "flat_kernel launch") ? 0 : -1;
+}
+
 template <int N>
 int launch_n(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, size_t bin0, size_t nb) {
   using S = FlatShape<N>;
   FlatArgs a;
   a.data = r.data;
   a.tmpl = r.tmpl;
-  a.twiddle = c->tw;
+  a.twiddle = c->twiddles(N);
+  if (!a.twiddle) return -1;
   a.data_stride = r.data_stride;
   a.tmpl_stride = r.tmpl_stride;
   a.nd = r.nd;
@@ -53,33 +69,34 @@ int launch_n(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, si
   // [bin0, bin0 + nb) of each row.
   a.out_idx = r.out_idx + bin0;
   a.out_val = r.out_val + bin0;
+  a.list = r.list;
+  a.list_count = r.list_count;
+  const bool list = r.list != nullptr;
   int dpw = c->tune_dpw;
-  if (dpw <= 0) dpw = std::min(r.nd, 16);
+  if (dpw <= 0) dpw = std::min(r.nd, 4);
   a.dpw = dpw;
-  dim3 grid((unsigned)((r.nt + S::SLOTS - 1) / S::SLOTS), (unsigned)((r.nd + dpw - 1) / dpw));
+  dim3 grid;
+  if (list) {
+    // enough workgroups to fill the device; each drains the list grid-stride
+    const int want = (r.list_cap + S::SLOTS - 1) / S::SLOTS;
+    grid = dim3((unsigned)std::max(1, std::min(want, c->refine_grid_cap)), 1);
+  } else {
+    grid = dim3((unsigned)((r.nt + S::SLOTS - 1) / S::SLOTS), (unsigned)((r.nd + dpw - 1) / dpw));
+    if (grid.y > 65535) return fail("too many data rows per launch (%d)", r.nd);
+  }
   const size_t lds = flat_lds_bytes<N>(nb);
   int mode = kGeneral;
   if (nb == 1) mode = kOneBin;
   else if (ws % 64 == 0 && (we % 64 == 0) && r.binsize % 64 == 0) mode = kAligned;
-  const void *fn = mode == kOneBin ? (const void *)flat_kernel<N, kOneBin>
-                 : mode == kAligned ? (const void *)flat_kernel<N, kAligned>
-                                    : (const void *)flat_kernel<N, kGeneral>;
-  static size_t lds_set[3] = {0, 0, 0};
-  if (lds > lds_set[mode]) {
-    if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                "hipFuncSetAttribute"))
-      return -1;
-    lds_set[mode] = lds;
+  Ctx::ProfScope ps(c, list ? "refine" : r.prof_name);
+  if (list) {
+    if (mode == kOneBin) return launch_mode<N, kOneBin, true>(c, a, grid, lds);
+    if (mode == kAligned) return launch_mode<N, kAligned, true>(c, a, grid, lds);
+    return launch_mode<N, kGeneral, true>(c, a, grid, lds);
   }
-  if (grid.y > 65535) return fail("too many data rows per launch (%d)", r.nd);
-  Ctx::ProfScope ps(c, "flat");
-  if (mode == kOneBin)
-    hipLaunchKernelGGL((flat_kernel<N, kOneBin>), grid, dim3(S::WG), lds, c->stream, a);
-  else if (mode == kAligned)
-    hipLaunchKernelGGL((flat_kernel<N, kAligned>), grid, dim3(S::WG), lds, c->stream, a);
-  else
-    hipLaunchKernelGGL((flat_kernel<N, kGeneral>), grid, dim3(S::WG), lds, c->stream, a);
-  return hip_ok(hipGetLastError(), "flat_kernel launch") ? 0 : -1;
+  if (mode == kOneBin) return launch_mode<N, kOneBin, false>(c, a, grid, lds);
+  if (mode == kAligned) return launch_mode<N, kAligned, false>(c, a, grid, lds);
+  return launch_mode<N, kGeneral, false>(c, a, grid, lds);
 }
 
 template <int N>
@@ -111,7 +128,7 @@ __global__ void twiddle_pass_kernel(cf *out, int ns) {
 }
 
 template <int N>
-int fill_twiddles(Ctx *c) {
+int fill_twiddles(Ctx *c, cf *tw) {
   using L = TwiddleLayout<N>;
   using S = FlatShape<N>;
   int ns = L::NS0;
@@ -119,7 +136,7 @@ int fill_twiddles(Ctx *c) {
     if (ns == 1) continue;
     const int cnt = ns * 4;
     hipLaunchKernelGGL(twiddle_pass_kernel, dim3((cnt + 255) / 256), dim3(256), 0, c->stream,
-                       c->tw + L::offset(p), ns);
+                       tw + L::offset(p), ns);
   }
   return hip_ok(hipGetLastError(), "twiddle launch") ? 0 : -1;
 }
@@ -141,18 +158,18 @@ size_t flat_twiddle_count(size_t n) {
   }
 }
 
-int flat_fill_twiddles(Ctx *c) {
-  switch (c->n) {
-    case 64: return fill_twiddles<64>(c);
-    case 128: return fill_twiddles<128>(c);
-    case 256: return fill_twiddles<256>(c);
-    case 512: return fill_twiddles<512>(c);
-    case 1024: return fill_twiddles<1024>(c);
-    case 2048: return fill_twiddles<2048>(c);
-    case 4096: return fill_twiddles<4096>(c);
-    case 8192: return fill_twiddles<8192>(c);
-    case 16384: return fill_twiddles<16384>(c);
-    default: return fail("transform length %zu is not supported", c->n);
+int flat_fill_twiddles(Ctx *c, size_t n, cf *tw) {
+  switch (n) {
+    case 64: return fill_twiddles<64>(c, tw);
+    case 128: return fill_twiddles<128>(c, tw);
+    case 256: return fill_twiddles<256>(c, tw);
+    case 512: return fill_twiddles<512>(c, tw);
+    case 1024: return fill_twiddles<1024>(c, tw);
+    case 2048: return fill_twiddles<2048>(c, tw);
+    case 4096: return fill_twiddles<4096>(c, tw);
+    case 8192: return fill_twiddles<8192>(c, tw);
+    case 16384: return fill_twiddles<16384>(c, tw);
+    default: return fail("transform length %zu is not supported", n);
   }
 }
 
@@ -165,14 +182,14 @@ bool flat_supported(size_t n) {
   }
 }
 
-int launch_flat(Ctx *c, const FlatRun &r) {
+int launch_flat(Ctx *c, size_t n, const FlatRun &r) {
   if (r.binsize < 1) return fail("binsize must be >= 1");
-  size_t we = std::min(r.we, c->n);
+  size_t we = std::min(r.we, n);
   if (r.ws >= we) return fail("empty window (%zu, %zu)", r.ws, r.we);
   FlatRun q = r;
   q.we = we;
   hipSetDevice(c->dev);
-  switch (c->n) {
+  switch (n) {
     case 64: return launch_len<64>(c, q);
     case 128: return launch_len<128>(c, q);
     case 256: return launch_len<256>(c, q);
@@ -182,7 +199,7 @@ int launch_flat(Ctx *c, const FlatRun &r) {
     case 4096: return launch_len<4096>(c, q);
     case 8192: return launch_len<8192>(c, q);
     case 16384: return launch_len<16384>(c, q);
-    default: return fail("transform length %zu is not supported", c->n);
+    default: return fail("transform length %zu is not supported", n);
   }
 }
 

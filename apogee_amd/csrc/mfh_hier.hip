@@ -1,0 +1,235 @@
+// Hierarchical matched filter on the device: coarse band screen, gate with
+// survivor compaction, and a list-driven refine -- one stream, no host round
+// trip between the stages.
+//
+// Reference behaviour (paths relative to the reference checkout):
+//   coarse templates T[:m]/sqrt(f), f in FP64     src/hmf.c:118-156,
+//                                                 python/matchedfilter/__init__.py:1284-1316
+//   coarse window, gate, refine, dismissal -1/0   src/hmf.c:222-328
+//   GPU precedent: coarse pass, compactPairs,
+//   indirect refine                               src/gpu/tierb.slang:986-1034,
+//                                                 python/matchedfilter/_vkcompute.py:650-850
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "mfh_dft.h"
+#include "mfh_internal.h"
+#include "mfhip.h"
+
+namespace mfh {
+
+namespace {
+
+// One workgroup per template: FP64 band fraction and the scaled coarse
+// template ct[k] = (float)(T[k] * scale), k < band, scale = f > 0 ? 1/sqrt(f)
+// : 0, with f = sum_{k<band}|T|^2 / sum |T|^2, or the bank-wide `ref_f` when
+// it is >= 0 (a reference profile was set).
+__global__ void __launch_bounds__(256) coarse_template_kernel(const cf *tmpl, long long n, int band, double ref_f,
+                                                              cf *ct) {
+  const int t = blockIdx.x;
+  const cf *T = tmpl + (long long)t * n;
+  __shared__ double part[2][256];
+  double f = ref_f;
+  if (ref_f < 0.0) {
+    double low = 0.0, tot = 0.0;
+    for (long long k = threadIdx.x; k < n; k += 256) {
+      const double re = T[k].x, im = T[k].y;
+      const double p = re * re + im * im;
+      tot += p;
+      if (k < band) low += p;
+    }
+    part[0][threadIdx.x] = low;
+    part[1][threadIdx.x] = tot;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) {
+        part[0][threadIdx.x] += part[0][threadIdx.x + s];
+        part[1][threadIdx.x] += part[1][threadIdx.x + s];
+      }
+      __syncthreads();
+    }
+    f = part[1][0] > 0.0 ? part[0][0] / part[1][0] : 0.0;
+  }
+  const double scale = f > 0.0 ? 1.0 / sqrt(f) : 0.0;
+  for (int k = threadIdx.x; k < band; k += 256)
+    ct[(long long)t * band + k] = cf{(float)((double)T[k].x * scale), (float)((double)T[k].y * scale)};
+}
+
+__global__ void zero_count_kernel(int *count) {
+  if (threadIdx.x == 0) *count = 0;
+}
+
+// Gate one pair on its coarse peak: fire iff a peak was reported (|z|^2 above
+// thr^2) and sqrt(|z|^2) >= thr, exactly the CPU engine's two tests.  Fired
+// pairs are appended to the refine list; the rest get -1 / 0 in every bin.
+__global__ void gate_kernel(const int64_t *cidx, const cf *cval, int nd, int nt, float thr, int nb,
+                            int64_t *out_idx, cf *out_val, int2 *list, int *count) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (long long)nd * nt) return;
+  bool fire = false;
+  if (cidx[p] >= 0) {
+    const cf z = cval[p];
+    const float m2 = __builtin_fmaf(z.x, z.x, z.y * z.y);
+    fire = sqrtf(m2) >= thr;
+  }
+  if (fire) {
+    const int slot = atomicAdd(count, 1);
+    list[slot] = int2{(int)(p / nt), (int)(p % nt)};
+  } else {
+    for (int b = 0; b < nb; b++) {
+      out_idx[p * nb + b] = -1;
+      out_val[p * nb + b] = cf{0.f, 0.f};
+    }
+  }
+}
+
+}  // namespace
+
+int hier_prepare(Ctx *c) {
+  if (!c->coarse_dirty) return 0;
+  const int band = c->coarse_band;
+  cf *ct = (cf *)c->scratch("coarse_tmpl", (size_t)c->nt * band * sizeof(cf));
+  if (!ct) return -1;
+  hipLaunchKernelGGL(coarse_template_kernel, dim3(c->nt), dim3(256), 0, c->stream, c->tmpl, (long long)c->n, band,
+                     c->coarse_ref_f, ct);
+  if (!hip_ok(hipGetLastError(), "coarse_template_kernel")) return -1;
+  c->coarse_tmpl = ct;
+  c->coarse_dirty = false;
+  return 0;
+}
+
+int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,
+             float coarse_thr, int64_t *idx, cf *val, int *count_out) {
+  if (c->coarse_band <= 0) return fail("no coarse band: call mfh_hier_set_band first");
+  if (!(coarse_thr >= 0.f) || !std::isfinite(coarse_thr))
+    return fail("coarse threshold must be finite and nonnegative");
+  if (d0 < 0 || t0 < 0 || nd < 1 || nt < 1 || d0 + nd > c->nd || t0 + nt > c->nt)
+    return fail("data/templates sub-range out of bounds");
+  if (binsize < 1) return fail("binsize must be >= 1");
+  if (we > c->n) we = c->n;
+  if (ws >= we) return fail("empty window (%zu, %zu)", ws, we);
+  hipSetDevice(c->dev);
+  if (hier_prepare(c) < 0) return -1;
+  const size_t n = c->n, m = (size_t)c->coarse_band;
+  const size_t nb = (we - ws + binsize - 1) / binsize;
+  const size_t pairs = (size_t)nd * nt;
+  // coarse window on the decimated grid (src/hmf.c:233-239)
+  const size_t R = n / m;
+  size_t cs = ws / R, ce = std::min(m, (we + R - 1) / R);
+  if (cs > 0) cs--;
+  int64_t *cidx = (int64_t *)c->scratch("coarse_idx", pairs * sizeof(int64_t));
+  cf *cval = (cf *)c->scratch("coarse_val", pairs * sizeof(cf));
+  int2 *list = (int2 *)c->scratch("refine_list", pairs * sizeof(int2));
+  int *count = (int *)c->scratch("refine_count", sizeof(int));
+  if (!cidx || !cval || !list || !count) return -1;
+
+  FlatRun cr;
+  cr.data = c->data + (size_t)d0 * n;
+  cr.tmpl = c->coarse_tmpl + (size_t)t0 * m;
+  cr.data_stride = (long long)n;
+  cr.tmpl_stride = (long long)m;
+  cr.nd = nd;
+  cr.nt = nt;
+  cr.binsize = ce - cs;
+  cr.threshold = coarse_thr;
+  cr.ws = cs;
+  cr.we = ce;
+  cr.out_idx = cidx;
+  cr.out_val = cval;
+  cr.prof_name = "coarse";
+  if (launch_flat(c, m, cr) < 0) return -1;
+
+  hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(64), 0, c->stream, count);
+  {
+    Ctx::ProfScope ps(c, "gate");
+    hipLaunchKernelGGL(gate_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, c->stream, cidx, cval, nd,
+                       nt, coarse_thr, (int)nb, idx, val, list, count);
+  }
+  if (!hip_ok(hipGetLastError(), "gate_kernel")) return -1;
+
+  FlatRun rr;
+  rr.data = c->data + (size_t)d0 * n;
+  rr.tmpl = c->tmpl + (size_t)t0 * n;
+  rr.data_stride = (long long)n;
+  rr.tmpl_stride = (long long)n;
+  rr.nd = nd;
+  rr.nt = nt;
+  rr.binsize = binsize;
+  rr.threshold = threshold;
+  rr.ws = ws;
+  rr.we = we;
+  rr.out_idx = idx;
+  rr.out_val = val;
+  rr.list = list;
+  rr.list_count = count;
+  rr.list_cap = (int)pairs;
+  if (launch_flat(c, n, rr) < 0) return -1;
+  if (count_out &&
+      !hip_ok(hipMemcpyAsync(count_out, count, sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy count"))
+    return -1;
+  return 0;
+}
+
+}  // namespace mfh
+
+using mfh::Ctx;
+using mfh::fail;
+using mfh::hip_ok;
+
+extern "C" int mfh_hier_set_band(mfh_ctx *ctx, size_t band, double ref_fraction) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (band < 64 || band >= c->n || (band & (band - 1)) || !mfh::flat_supported(band))
+    return fail("band must be a power of two, >= 64 and < n (got %zu)", band);
+  if (ref_fraction > 1.0 || !(ref_fraction == ref_fraction)) return fail("reference band fraction out of range");
+  c->coarse_band = (int)band;
+  c->coarse_ref_f = ref_fraction < 0.0 ? -1.0 : ref_fraction;
+  c->coarse_dirty = true;
+  return 0;
+}
+
+extern "C" int mfh_hier_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
+                                    size_t ws, size_t we, float coarse_thr, int64_t *dev_idx, void *dev_val_c64,
+                                    int32_t *host_refined) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  return mfh::hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, dev_idx, (mfh::cf *)dev_val_c64,
+                       host_refined);
+}
+
+extern "C" long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
+                               size_t ws, size_t we, float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
+                               int64_t *n_refined) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (!idx || !val_c64) return fail("null output pointer");
+  const size_t nb = mfh_nbins(c->n, binsize, ws, we);
+  if (binsize < 1) return fail("binsize must be >= 1");
+  if (nb == 0) return fail("empty window (%zu, %zu)", ws, we);
+  if (nd < 1 || nt < 1) return fail("data/templates sub-range out of bounds");
+  const size_t cells = (size_t)nd * (size_t)nt * nb;
+  hipSetDevice(c->dev);
+  int64_t *didx = (int64_t *)c->scratch("out_idx", cells * sizeof(int64_t));
+  mfh::cf *dval = (mfh::cf *)c->scratch("out_val", cells * sizeof(mfh::cf));
+  if (!didx || !dval) return -1;
+  int refined = 0;
+  if (mfh::hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, didx, dval, &refined) < 0) return -1;
+  if (!hip_ok(hipMemcpyAsync(idx, didx, cells * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream), "copy idx") ||
+      !hip_ok(hipMemcpyAsync(val_c64, dval, cells * sizeof(mfh::cf), hipMemcpyDeviceToHost, c->stream),
+              "copy val") ||
+      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+    return -1;
+  c->prof_collect();
+  if (n_refined) *n_refined = refined;
+  long total = 0;
+  for (size_t p = 0; p < (size_t)nd * nt; p++) {
+    int cnt = 0;
+    for (size_t b = 0; b < nb; b++) cnt += idx[p * nb + b] >= 0;
+    if (counts) counts[p] = cnt;
+    total += cnt;
+  }
+  return total;
+}

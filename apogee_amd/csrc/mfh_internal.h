@@ -21,8 +21,18 @@ struct Ctx {
   hipStream_t stream = nullptr;
   cf *data = nullptr;  // [nd][n]
   cf *tmpl = nullptr;  // [nt][n]
-  cf *tw = nullptr;    // per-pass twiddle tables of the flat kernel
-  int tune_dpw = 0;    // 0: automatic
+  int tune_dpw = 0;          // 0: automatic
+  // hierarchical state: coarse band, bank-wide reference fraction (< 0: per
+  // template), scaled coarse templates [nt][band], rebuilt when dirty
+  int coarse_band = 0;
+  double coarse_ref_f = -1.0;
+  bool coarse_dirty = true;
+  cf *coarse_tmpl = nullptr;
+  int refine_grid_cap = 2048;  // workgroups of a list-driven (refine) launch
+  // Per-pass twiddle tables of the flat kernel for length n (FP64-generated
+  // on first use, kept for the context's lifetime).
+  std::map<size_t, cf *> tw_by_n;
+  const cf *twiddles(size_t n);
 
   struct Buf {
     void *p = nullptr;
@@ -63,15 +73,19 @@ struct FlatRun {
   size_t ws = 0, we = 0;
   int64_t *out_idx = nullptr;
   cf *out_val = nullptr;
-  long long out_row_stride = 0;  // unused (dense [nd][nt][nb])
+  // list mode: pairs (data row, template row) read from the device
+  const int2 *list = nullptr;
+  const int *list_count = nullptr;
+  int list_cap = 0;
+  const char *prof_name = "flat";
 };
 
 bool flat_supported(size_t n);
 // Per-pass twiddle table size (complex elements) and its FP64 generation.
 size_t flat_twiddle_count(size_t n);
-int flat_fill_twiddles(Ctx *c);
+int flat_fill_twiddles(Ctx *c, size_t n, cf *tw);
 // Validates, splits windows whose bin table would not fit LDS, and launches
-// on c->stream.  Returns 0 or -1 (message set).
-int launch_flat(Ctx *c, const FlatRun &r);
+// the length-n flat kernel on c->stream.  Returns 0 or -1 (message set).
+int launch_flat(Ctx *c, size_t n, const FlatRun &r);
 
 }  // namespace mfh

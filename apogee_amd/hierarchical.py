@@ -1,9 +1,176 @@
-"""HierarchicalFilter: coarse band screen on the device, then exact refine."""
-from . import MatchedFilter
+"""HierarchicalFilter: a coarse band screen on the device, then the exact
+flat filter on the pairs that survive it.
+
+Mirrors python/matchedfilter/__init__.py:1055-1461 of the reference
+(constructor validation, ``set_reference``, ``set_coarse_threshold``,
+``set_first_stage``, ``config``, ``stats``, ``refine_rate``).  Execution is
+one device call per run (libmfhip ``mfh_hier_peaks``): the coarse templates
+T[:band]/sqrt(f) are built on the device with FP64 band fractions, the coarse
+pass is the band-point flat filter over the coarse window, the gate compacts
+survivors into a device list and the refine kernel drains it; dismissed pairs
+report -1 / 0.  Every reported peak is the flat filter's peak; only omissions
+are possible.
+
+Configuration: an explicit ``band`` (and optionally ``set_coarse_threshold``)
+is used as given.  Otherwise the band comes from this build's gfx950 band
+table and the coarse gate from the reference profile through
+``apogee_amd.gatemodel.gate_for`` at (snr, fd), which needs
+``set_reference``.
+"""
+import numpy as np
+
+from . import MatchedFilter, _from_any
+
+#: Coarse band per transform length on gfx950 (power of two, 64 <= band < n).
+#: At n = 4096 the 512-point band is what the benchmark configuration pins.
+_GFX950_BAND = {1024: 128, 2048: 256, 4096: 512, 8192: 1024, 16384: 2048,
+                32768: 4096, 65536: 8192}
 
 
 class HierarchicalFilter(MatchedFilter):
-    """Placeholder until the coarse screen lands."""
+    """Matched filter that screens pairs on the low band first.
 
-    def __init__(self, *a, **k):
-        raise NotImplementedError("HierarchicalFilter is not built yet")
+    ``snr`` is the |rho| of the weakest signal that must be kept and ``fd`` the
+    tolerated false-dismissal probability for it.  Compare values across
+    devices within float32 roundoff; the coarse gate can omit peaks, never
+    invent or move one.
+    """
+
+    def __init__(self, n, ndata=1, ntemplates=1, snr=5.5, fd=1e-2,
+                 band=None, taps=None, device=None):
+        n = int(n)
+        if band is not None:
+            band = int(band)
+            if band < 64 or band >= n or band & (band - 1):
+                raise ValueError("band must be a power of two, >= 64 and < n")
+        if taps is not None:
+            taps = int(taps)
+            if taps < 2 or taps > 64 or taps % 2:
+                raise ValueError("taps must be even and between 2 and 64")
+        super().__init__(n, ndata, ntemplates, device)
+        self.snr = float(snr)
+        self.fd = float(fd)
+        self._pinned = (band, int(taps or 8)) if band is not None else None
+        self._ref = None
+        self._cal_thr = None
+        self._fs_snr = None
+        self._cal = None
+        self._band_set = None
+        self._pairs = 0
+        self._trig = 0
+
+    # ----------------------------------------------------------- settings
+    def set_reference(self, power):
+        """Reference output-power profile (length n, any scale), or None."""
+        if power is not None:
+            p = np.ascontiguousarray(_from_any(power), dtype=np.float32)
+            if p.shape != (self.n,):
+                raise ValueError("reference must be a one-dimensional array of length %d" % self.n)
+            if not np.isfinite(p).all() or np.any(p < 0) or not np.any(p > 0):
+                raise ValueError("reference must be finite, nonnegative, with positive total power")
+            self._ref = p
+        else:
+            self._ref = None
+        self._cal = None
+
+    def set_coarse_threshold(self, value):
+        """Set the coarse gate directly (requires an explicit band); None
+        returns to the reference model."""
+        if value is None:
+            self._cal_thr = None
+            self._cal = None
+            return
+        value = float(value)
+        if not np.isfinite(value) or value < 0 or value > float(np.finfo(np.float32).max):
+            raise ValueError("coarse threshold must be finite nonnegative float32, or None")
+        if self._pinned is None:
+            raise ValueError("an explicit coarse threshold requires an explicit band")
+        self._cal_thr = value
+        self._cal = None
+
+    def set_first_stage(self, snr):
+        """Calibrate the coarse gate at ``snr`` instead of the constructor's;
+        None or a non-positive value restores it."""
+        self._fs_snr = float(snr) if snr is not None and float(snr) > 0 else None
+        self._cal = None
+
+    # ------------------------------------------------------- calibration
+    def _band(self):
+        if self._pinned is not None:
+            return self._pinned[0]
+        if self._ref is None:
+            raise ValueError("set_reference is required for automatic configuration selection")
+        band = _GFX950_BAND.get(self.n)
+        if band is None:
+            raise ValueError("no gfx950 band for n=%d: pass band= explicitly" % self.n)
+        return band
+
+    def _calibration(self):
+        key = (self.snr, self.fd, self._fs_snr, self._pinned, self._cal_thr,
+               None if self._ref is None else self._ref.tobytes())
+        if self._cal is not None and self._cal[0] == key:
+            return self._cal[1]
+        band = self._band()
+        if self._cal_thr is not None:
+            thr = float(self._cal_thr)
+        else:
+            if self._ref is None:
+                raise ValueError(
+                    "no calibrated coarse threshold for n=%d band=%d: provide a reference "
+                    "or set both band and coarse threshold" % (self.n, band))
+            from .gatemodel import gate_for
+            thr = gate_for(self._ref, self.n, band, self._fs_snr or self.snr, self.fd)
+            if thr is None:
+                raise ValueError("the false-dismissal budget fd=%g cannot be resolved" % self.fd)
+            if not np.isfinite(thr) or thr < 0 or thr > float(np.finfo(np.float32).max):
+                raise ValueError("calibrated coarse threshold must be finite, nonnegative float32")
+        f = -1.0
+        if self._ref is not None:
+            ref = self._ref.astype(np.float64)
+            f = float(ref[:band].sum() / ref.sum())
+        out = (band, f, thr)
+        self._cal = (key, out)
+        return out
+
+    def _dispatch(self, d0, nd, t0, nt, binsize, threshold, start, end, counts):
+        band, f, thr = self._calibration()
+        if self._band_set != (band, f):
+            self._gpu.hier_set_band(band, f)
+            self._band_set = (band, f)
+        idx, val, cnt, refined = self._gpu.hier_peaks(d0, nd, t0, nt, binsize, float(threshold),
+                                                      start, end, thr, counts=bool(counts))
+        self._pairs += nd * nt
+        self._trig += int(refined)
+        return idx, val, cnt
+
+    def _on_ingest(self, kind, index, a):
+        if kind == "template":
+            self._band_set = None
+
+    # ----------------------------------------------------------- reports
+    @property
+    def config(self):
+        """Selected or pinned ``(band, taps)``."""
+        if self._pinned is not None:
+            return self._pinned
+        return (self._band(), 8)
+
+    @property
+    def cost_table(self):
+        """Which band table chose the band: None when pinned."""
+        return None if self._pinned is not None else "gfx950"
+
+    @property
+    def coarse_threshold(self):
+        """The coarse gate in use (resolving it if needed)."""
+        return self._calibration()[2]
+
+    @property
+    def stats(self):
+        """``(pairs, triggers)`` accumulated since construction."""
+        return (self._pairs, self._trig)
+
+    @property
+    def refine_rate(self):
+        """Fraction of pairs that needed the full correlation."""
+        return self._trig / self._pairs if self._pairs else 0.0

@@ -105,6 +105,43 @@ int mfh_flat_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
                          size_t binsize, float threshold, size_t ws, size_t we,
                          int64_t *dev_idx, void *dev_val_c64);
 
+/* ---- hierarchical filter ------------------------------------------------ */
+
+/* Select the coarse band (a power of two, 64 <= band < n) and how coarse
+ * templates are normalised: ref_fraction in [0, 1] is the bank-wide in-band
+ * fraction of a reference profile (sum_{k<band} ref / sum ref, FP64 on the
+ * host); a negative value normalises each template by its own FP64 band
+ * fraction, computed on the device.  Coarse templates T[:band]/sqrt(f) are
+ * rebuilt on the device whenever templates or this setting change.
+ * Replaces: ap_hmf_create_ex band / ap_hmf_set_reference
+ * (python/matchedfilter/matchedfilter.h:129-154; src/hmf.c:118-156) and the
+ * host-side coarse-template scaling of HierarchicalFilter._gpu_dispatch
+ * (python/matchedfilter/__init__.py:1284-1303). */
+int mfh_hier_set_band(mfh_ctx *ctx, size_t band, double ref_fraction);
+
+/* Hierarchical run: coarse band screen (the band-point flat filter on
+ * D[:band] and the coarse templates, one bin over the coarse window), gate
+ * (fire iff coarse |z|^2 > coarse_thr^2 and |z| >= coarse_thr), survivor
+ * compaction, and the exact flat filter on survivors; dismissed pairs report
+ * -1 / 0 in every bin.  Reported peaks are those of mfh_flat_peaks.  Outputs
+ * and counts as mfh_flat_peaks; *n_refined receives the number of pairs that
+ * went to the full filter.  Returns total crossings, or < 0.
+ * Replaces: Context.hier_peaks (python/matchedfilter/_vkcompute.py:534-623)
+ * and ap_hmf_run (matchedfilter.h:159-161; src/hmf.c:222-328). */
+long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
+                    size_t binsize, float threshold, size_t ws, size_t we,
+                    float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
+                    int64_t *n_refined);
+
+/* Same with device outputs, queued on the stream; if host_refined is not
+ * NULL it receives the survivor count once the stream reaches that point.
+ * Replaces: the one-command-buffer hierarchical recording of
+ * Context._make_hier (python/matchedfilter/_vkcompute.py:650-850). */
+int mfh_hier_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
+                         size_t binsize, float threshold, size_t ws, size_t we,
+                         float coarse_thr, int64_t *dev_idx, void *dev_val_c64,
+                         int32_t *host_refined);
+
 /* Tuning knobs: key "dpw" (data rows per workgroup; 0 = automatic).
  * Replaces: MF_MFTILE / MF_PBMAX environment knobs (src/matchfilt.c:93-107). */
 int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value);

@@ -1,0 +1,56 @@
+"""A host double of apogee_amd._hip.Context, for testing the Python layer on
+machines without a GPU.  It answers with the FP64 oracle (values cast to
+complex64); it is test infrastructure and never reachable from the package.
+"""
+import numpy as np
+
+from oracle import mf_oracle as O
+
+
+class FakeContext:
+    calls = []
+
+    def __init__(self, index, n, ndata, ntemplates):
+        self.index, self.n = index, n
+        self.ndata, self.ntemplates = ndata, ntemplates
+        self.D = np.zeros((ndata, n), np.complex64)
+        self.T = np.zeros((ntemplates, n), np.complex64)
+        self.band = None
+        self.ref_f = -1.0
+        self.last_refinements = 0
+        FakeContext.calls.append(("create", n, ndata, ntemplates))
+
+    def set_rows(self, kind, row0, a):
+        a = np.asarray(a)
+        dst = self.D if kind == "data" else self.T
+        dst[row0:row0 + a.shape[0]] = a
+        FakeContext.calls.append(("set", kind, row0, a.shape[0]))
+
+    def peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, counts=False):
+        FakeContext.calls.append(("peaks", d0, nd, t0, nt, binsize, threshold, start, end))
+        idx, val, _ = O.flat_peaks(self.D[d0:d0 + nd], self.T[t0:t0 + nt], binsize, threshold,
+                                   (start, end))
+        cnt = (idx >= 0).sum(axis=2).astype(np.int32) if counts else None
+        return idx, val.astype(np.complex64), cnt
+
+    def hier_set_band(self, band, ref_fraction=-1.0):
+        self.band, self.ref_f = band, ref_fraction
+
+    def hier_peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, coarse_threshold,
+                   counts=False):
+        FakeContext.calls.append(("hier", d0, nd, t0, nt, coarse_threshold))
+        T = self.T[t0:t0 + nt]
+        if self.ref_f >= 0:
+            f = np.full(nt, self.ref_f)
+        else:
+            f = O.band_fraction(T, self.band)
+        sc = np.where(f > 0, 1 / np.sqrt(np.where(f > 0, f, 1)), 0)
+        ct = (T[:, :self.band].astype(np.complex128) * sc[:, None]).astype(np.complex64)
+        cs, ce = O.coarse_window(self.n, self.band, start, end)
+        zc = O.correlate(self.D[d0:d0 + nd, None, :self.band], ct[None])
+        cmax = np.abs(zc[..., cs:ce]).max(axis=-1)
+        idx, val, _, fired = O.hier_peaks(self.D[d0:d0 + nd], T, self.band, coarse_threshold,
+                                          binsize, threshold, (start, end), coarse_max=cmax)
+        cnt = (idx >= 0).sum(axis=2).astype(np.int32) if counts else None
+        self.last_refinements = int(fired.sum())
+        return idx, val.astype(np.complex64), cnt, int(fired.sum())

@@ -1,0 +1,212 @@
+"""The Python surface on a host double of the device (no GPU needed).
+
+Mirrors the reference's API contract (python/matchedfilter/__init__.py:176-686,
+:1055-1461; tests/test_api.py, tests/test_device_parity.py:31-140): shapes
+and errors, sub-range = slice, raw = structured, counts, readiness checks,
+hierarchical configuration, stats and the gate model's invariants.
+"""
+import numpy as np
+import pytest
+
+import apogee_amd as mf
+from apogee_amd import device as mfdev
+from oracle import mf_oracle as O
+from _fake import FakeContext
+
+
+@pytest.fixture
+def fake(monkeypatch):
+    monkeypatch.setattr(mf.MatchedFilter, "_context_factory", FakeContext)
+    monkeypatch.setattr(mfdev, "devices", lambda: [mfdev.Device("gpu", 0, "fake gfx950")])
+    FakeContext.calls = []
+    return FakeContext
+
+
+def cnoise(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+def test_shapes_and_errors(fake):
+    with pytest.raises(ValueError):
+        mf.MatchedFilter(1000, 1, 1)
+    with pytest.raises(ValueError):
+        mf.MatchedFilter(1024, 0, 1)
+    f = mf.MatchedFilter(1024, 2, 3)
+    with pytest.raises(ValueError):
+        f.set_data(np.zeros((2, 100), np.complex64))
+    with pytest.raises(IndexError):
+        f.set_data(np.zeros(1024, np.complex64), index=2)
+    with pytest.raises(ValueError):
+        f.set_data(np.zeros(100, np.complex64), index=0)
+    with pytest.raises(ValueError, match="no data"):
+        f.run()
+    f.set_data(np.zeros((2, 1024), np.complex64))
+    with pytest.raises(ValueError, match="no templates"):
+        f.run()
+    f.set_templates(np.zeros((3, 1024), np.complex64))
+    with pytest.raises(ValueError):
+        f.run(binsize=0)
+    with pytest.raises(ValueError):
+        f.run(window=(500, 500))
+    with pytest.raises(ValueError):
+        f.run(data=(1, 2))
+    with pytest.raises(ValueError):
+        f.nbins(0)
+    assert f.nbins(100) == 11
+    assert f.nbins(100, window=(10, 1000)) == 10
+
+
+def test_device_strings(fake):
+    assert str(mfdev.parse(None)) == "gpu:0"
+    assert str(mfdev.parse("gpu:0")) == "gpu:0"
+    assert mfdev.parse("auto") == "gpu:0"
+    with pytest.raises(ValueError):
+        mfdev.parse("cpu")
+    with pytest.raises(ValueError):
+        mfdev.parse("tpu")
+    with pytest.raises(ValueError):
+        mfdev.parse("gpu:7")
+    with pytest.raises(TypeError):
+        mfdev.parse(3)
+
+
+def test_results_layout_subranges_raw_counts(fake):
+    rng = np.random.default_rng(3)
+    n, nd, nt = 1024, 4, 3
+    D, T = cnoise(rng, (nd, n)), cnoise(rng, (nt, n))
+    f = mf.MatchedFilter(n, nd, nt)
+    f.set_data(D)
+    f.set_templates(T)
+    full = f.run(binsize=256, threshold=0.0, window=(100, 900))
+    assert full.dtype == mf.PEAK_DTYPE and full.shape == (nd, nt, 4)
+    ridx, rval, _ = O.flat_peaks(D, T, 256, 0.0, (100, 900))
+    np.testing.assert_array_equal(full["index"], ridx)
+    sub = f.run(binsize=256, threshold=0.0, window=(100, 900), data=(1, 2), templates=(1, 2))
+    np.testing.assert_array_equal(sub["index"], full["index"][1:3, 1:3])
+    idx, val = f.run(binsize=256, threshold=0.0, window=(100, 900), raw=True)
+    np.testing.assert_array_equal(idx, full["index"])
+    np.testing.assert_array_equal(val, full["value"])
+    pk, cnt = f.run(binsize=128, threshold=60.0, counts=True)
+    np.testing.assert_array_equal(cnt, (pk["index"] >= 0).sum(axis=2))
+    # the device saw exactly the requested sub-range
+    assert ("peaks", 1, 2, 1, 2, 256, 0.0, 100, 900) in fake.calls
+
+
+def test_window_is_clamped(fake):
+    rng = np.random.default_rng(4)
+    f = mf.MatchedFilter(512, 1, 1)
+    f.set_data(cnoise(rng, (1, 512)))
+    f.set_templates(cnoise(rng, (1, 512)))
+    pk = f.run(binsize=1000, window=(-50, 5000))
+    assert pk.shape == (1, 1, 1)
+    assert fake.calls[-1][-2:] == (0, 512)
+
+
+def test_single_rows_track_readiness(fake):
+    n = 512
+    f = mf.MatchedFilter(n, 3, 2)
+    f.set_templates(np.ones((2, n), np.complex64))
+    f.set_data(np.ones(n, np.complex64), index=0)
+    f.run(data=(0, 1))                     # row 0 is ready
+    with pytest.raises(ValueError, match="no data"):
+        f.run()                            # rows 1, 2 are not
+    f.set_data(np.ones(n, np.complex64), index=1)
+    f.set_data(np.ones(n, np.complex64), index=2)
+    f.run()
+
+
+def test_device_arrays_other_than_rocm_are_refused(fake):
+    class Foreign:
+        def __dlpack_device__(self):
+            return (2, 0)                  # CUDA
+    f = mf.MatchedFilter(512, 1, 1)
+    with pytest.raises(TypeError):
+        f.set_data(Foreign())
+
+
+# ----------------------------------------------------------- hierarchical
+
+def inspiral_power(n, exponent=-7 / 3.0, knee_frac=0.0150):
+    p = np.zeros(n, dtype=np.float32)
+    k = np.arange(1, n // 2).astype(np.float64)
+    p[1:n // 2] = (k ** exponent / ((knee_frac * n / k) ** 4 + 1.0)).astype(np.float32)
+    return p / p.sum()
+
+
+def test_hierarchical_validation_and_config(fake):
+    with pytest.raises(ValueError):
+        mf.HierarchicalFilter(4096, band=100)
+    with pytest.raises(ValueError):
+        mf.HierarchicalFilter(4096, band=4096)
+    with pytest.raises(ValueError):
+        mf.HierarchicalFilter(4096, band=512, taps=3)
+    h = mf.HierarchicalFilter(4096, 2, 2)
+    with pytest.raises(ValueError):
+        h.set_coarse_threshold(3.0)        # needs an explicit band
+    with pytest.raises(ValueError):
+        h.config                           # automatic band needs a reference
+    h.set_reference(inspiral_power(4096))
+    assert h.config == (512, 8) and h.cost_table == "gfx950"
+    with pytest.raises(ValueError):
+        h.set_reference(np.ones(10))
+    with pytest.raises(ValueError):
+        h.set_reference(-np.ones(4096))
+    p = mf.HierarchicalFilter(4096, 2, 2, band=256, taps=4)
+    assert p.config == (256, 4) and p.cost_table is None
+    with pytest.raises(ValueError):
+        p.set_coarse_threshold(float("nan"))
+    p.set_coarse_threshold(2.5)
+    assert p.coarse_threshold == 2.5
+
+
+def test_hierarchical_run_is_identical_or_absent_and_counts_triggers(fake):
+    rng = np.random.default_rng(11)
+    n, nd, nt = 1024, 4, 4
+    power = inspiral_power(n)
+    H = np.stack([(np.sqrt(power) * np.exp(2j * np.pi * rng.random(n))).astype(np.complex64)
+                  for _ in range(nt)])
+    H /= np.linalg.norm(H, axis=1, keepdims=True)
+    D = cnoise(rng, (nd, n))
+    D[1] += 12 * H[2]
+    flat = mf.MatchedFilter(n, nd, nt)
+    hier = mf.HierarchicalFilter(n, nd, nt, snr=5.5, fd=1e-2)
+    hier.set_reference(power)
+    for o in (flat, hier):
+        o.set_data(D)
+        o.set_templates(H)
+    a = flat.run(binsize=256, threshold=5.5)
+    b = hier.run(binsize=256, threshold=5.5)
+    fired = b["index"] >= 0
+    assert fired[1, 2].any()
+    np.testing.assert_array_equal(a["index"][fired], b["index"][fired])
+    pairs, trig = hier.stats
+    assert pairs == nd * nt and 1 <= trig < nd * nt
+    assert hier.refine_rate == trig / pairs
+    assert 0 < hier.coarse_threshold < 5.5
+
+
+# -------------------------------------------------------------- gate model
+
+def test_gate_model_invariants():
+    from apogee_amd import gatemodel as G
+    import os
+    p = np.load(os.path.join(os.path.dirname(__file__), "data", "reference_profile_pycbc.npy"),
+                allow_pickle=False)
+    g2 = G.gate_for(p, 4096, 512, 5.5, 1e-2)
+    g3 = G.gate_for(p, 4096, 512, 5.5, 1e-3)
+    assert 0 < g3 < g2 < 5.5                       # a tighter budget lowers the gate
+    assert G.gate_for(p, 4096, 512, 6.5, 1e-2) > g2  # a stronger signal raises it
+    # the modelled dismissal at the gate is the budget
+    assert G.dismissal(p, 4096, 512, 5.5, g2, 1e-2) == pytest.approx(1e-2, abs=2e-3)
+    with pytest.raises(ValueError):
+        G.gate_for(p, 4096, 500, 5.5, 1e-2)
+    with pytest.raises(ValueError):
+        G.gate_for(p, 4096, 512, 5.5, 1.5)
+    # the coarse signal is at most snr * sqrt(f): the gate sits below it
+    f = float(p[:512].sum() / p.sum())
+    assert g2 < 5.5 * np.sqrt(f)
+    # a profile wholly in band but narrow-peaked in lag scallops on the coarse
+    # grid, so its gate is lower than the broadband profile's
+    flat = np.zeros(4096)
+    flat[:512] = 1.0
+    assert G.gate_for(flat, 4096, 512, 5.5, 1e-2) < g2

@@ -1,0 +1,203 @@
+"""HierarchicalFilter on the MI355X.
+
+Asserted against the one-sided guarantee (the reference's
+tests/test_hierarchical_matrix.py idea): every reported peak is bit-identical
+to the flat filter's on the same device, only omissions are allowed; plus the
+coarse decisions against the FP64 oracle, the measured omission rate against
+the budget, and that the screen actually skips work on noise.
+"""
+import numpy as np
+import pytest
+
+import apogee_amd as mf
+from oracle import mf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+PROFILE = None
+
+
+def inspiral_power(n, exponent=-7 / 3.0, knee_frac=0.0150):
+    p = np.zeros(n, dtype=np.float32)
+    k = np.arange(1, n // 2).astype(np.float64)
+    p[1:n // 2] = (k ** exponent / ((knee_frac * n / k) ** 4 + 1.0)).astype(np.float32)
+    return p / p.sum()
+
+
+def template_with_power(n, power, rng=None):
+    h = np.sqrt(power).astype(np.complex64)
+    if rng is not None:   # random phases keep the bank's templates distinct
+        h = (h * np.exp(2j * np.pi * rng.random(n))).astype(np.complex64)
+    return h / np.linalg.norm(h)
+
+
+def noise(shape, rng):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+def loud_case(n, nd, nt, seed=11, snr=12.0):
+    rng = np.random.default_rng(seed)
+    power = inspiral_power(n)
+    H = np.stack([template_with_power(n, power, rng) for _ in range(nt)])
+    D = noise((nd, n), rng)
+    for d in range(nd):
+        lag = 300 + 17 * d
+        D[d] += (snr * H[d % nt] * np.exp(2j * np.pi * np.arange(n) * lag / n)).astype(np.complex64)
+    return power, D, H
+
+
+def build(n, nd, nt, power, snr=5.5, fd=1e-2, band=None):
+    flat = mf.MatchedFilter(n, nd, nt, device="gpu")
+    hier = mf.HierarchicalFilter(n, nd, nt, snr=snr, fd=fd, band=band, device="gpu")
+    hier.set_reference(power)
+    return flat, hier
+
+
+def assert_one_sided(a, b, context=""):
+    fired = b["index"] >= 0
+    assert not (fired & (a["index"] < 0)).any(), "invented a peak %s" % context
+    np.testing.assert_array_equal(a["index"][fired], b["index"][fired], err_msg=context)
+    np.testing.assert_array_equal(a["value"][fired], b["value"][fired], err_msg=context)
+    return fired
+
+
+def run_both(n, nd, nt, binsize, threshold, window, seed=11, snr=12.0):
+    power, D, H = loud_case(n, nd, nt, seed=seed, snr=snr)
+    flat, hier = build(n, nd, nt, power)
+    for o in (flat, hier):
+        o.set_data(D)
+        o.set_templates(H)
+    kw = {} if window is None else {"window": window}
+    return flat.run(binsize=binsize, threshold=threshold, **kw), \
+        hier.run(binsize=binsize, threshold=threshold, **kw), hier
+
+
+@pytest.mark.parametrize("binsize", [1, 3, 64, 100, 257, 512, 1024, 4096])
+def test_binsize(binsize, gpu_device):
+    a, b, _ = run_both(4096, 2, 4, binsize, 5.5, None)
+    assert_one_sided(a, b, "binsize=%d" % binsize)
+
+
+@pytest.mark.parametrize("window", [(0, 4096), (0, 4000), (37, 4000), (1, 4095), (500, 501),
+                                    (2048, 4096)])
+def test_window_alignment(window, gpu_device):
+    a, b, _ = run_both(4096, 2, 4, 128, 5.5, window)
+    assert_one_sided(a, b, "window=%s" % (window,))
+
+
+@pytest.mark.parametrize("binsize,window", [(300, (37, 4000)), (7, (13, 3999)), (8192, (100, 900))])
+def test_awkward_combinations(binsize, window, gpu_device):
+    a, b, _ = run_both(4096, 2, 4, binsize, 5.5, window)
+    assert_one_sided(a, b, "binsize=%d window=%s" % (binsize, window))
+
+
+@pytest.mark.parametrize("n", [1024, 2048, 4096, 8192, 16384])
+def test_every_transform_length(n, gpu_device):
+    a, b, _ = run_both(n, 2, 4, n // 8, 5.5, None)
+    assert_one_sided(a, b, "n=%d" % n)
+
+
+@pytest.mark.parametrize("threshold", [3.0, 5.5, 8.0, 20.0])
+def test_thresholds(threshold, gpu_device):
+    a, b, _ = run_both(4096, 2, 4, 512, threshold, None)
+    fired = assert_one_sided(a, b, "threshold=%.1f" % threshold)
+    if threshold >= 20.0:
+        assert not fired.any()
+
+
+def test_loud_signals_survive_and_noise_is_screened(gpu_device):
+    a, b, hier = run_both(4096, 4, 4, 1024, 5.5, None)
+    loud = np.zeros((4, 4), bool)
+    for d in range(4):
+        loud[d, d % 4] = True
+    assert (b["index"][loud] >= 0).any(axis=-1).all(), "a 12-sigma signal was dismissed"
+    n, nd, nt = 4096, 4, 16
+    rng = np.random.default_rng(5)
+    power = inspiral_power(n)
+    H = np.stack([template_with_power(n, power, rng) for _ in range(nt)])
+    _, hier = build(n, nd, nt, power)
+    hier.set_data(noise((nd, n), rng))
+    hier.set_templates(H)
+    hier.run(binsize=n, threshold=5.5)
+    assert hier.refine_rate < 0.5, hier.refine_rate
+
+
+def test_coarse_decisions_match_the_fp64_oracle(gpu_device):
+    """The device gate reproduces the oracle's fired set except where the
+    FP64 coarse maximum sits within 1e-4 (relative) of the gate."""
+    n, nd, nt, band = 4096, 8, 32, 512
+    power, D, H = loud_case(n, nd, nt, seed=3, snr=7.0)
+    hier = mf.HierarchicalFilter(n, nd, nt, band=band, device="gpu")
+    hier.set_reference(power)
+    gate = 4.0
+    hier.set_coarse_threshold(gate)
+    hier.set_data(D)
+    hier.set_templates(H)
+    got = hier.run(binsize=n, threshold=0.0)
+    f = O.band_fraction(H, band, power)
+    ct = (H[:, :band].astype(np.complex128) / np.sqrt(f)).astype(np.complex64)
+    zc = O.correlate(D[:, None, :band], ct[None, :, :])
+    cmax = np.abs(zc).max(axis=-1)
+    fired_oracle = cmax >= gate
+    fired_dev = got["index"][..., 0] >= 0
+    near = np.abs(cmax - gate) <= 1e-4 * gate
+    assert ((fired_oracle == fired_dev) | near).all()
+    assert hier.stats == (nd * nt, int(fired_dev.sum()))
+
+
+def test_explicit_threshold_zero_refines_everything(gpu_device):
+    n = 4096
+    power, D, H = loud_case(n, 8, 4, seed=5, snr=6.0)
+    rates = {}
+    for thr in (0.0, 3.5, 4.5):
+        hf = mf.HierarchicalFilter(n, 8, 4, snr=5.0, fd=1e-3, band=512, taps=8, device="gpu")
+        hf.set_reference(power)
+        hf.set_coarse_threshold(thr)
+        hf.set_templates(H)
+        hf.set_data(D)
+        hf.run(binsize=n, threshold=5.0)
+        rates[thr] = hf.refine_rate
+    assert rates[0.0] == pytest.approx(1.0)
+    assert rates[3.5] >= rates[4.5]
+
+
+def test_omission_rate_meets_the_budget(gpu_device):
+    """Measured false dismissal of SNR-5.5 injections against fd = 1e-2."""
+    n, trials, snr, fd = 4096, 4096, 5.5, 1e-2
+    rng = np.random.default_rng(13)
+    power = inspiral_power(n)
+    H = template_with_power(n, power)
+    nd = 512
+    flat, hier = build(n, nd, 1, power, snr=snr, fd=fd)
+    flat.set_templates(H[None, :])
+    hier.set_templates(H[None, :])
+    ph = np.exp(2j * np.pi * np.arange(n) / n)
+    detected = omitted = 0
+    for b0 in range(0, trials, nd):
+        D = noise((nd, n), rng)
+        for i in range(nd):
+            D[i] += (snr * H * ph ** ((37 * (b0 + i)) % n)).astype(np.complex64)
+        flat.set_data(D)
+        hier.set_data(D)
+        a = flat.run(binsize=n, threshold=snr)["index"][:, 0, 0]
+        b = hier.run(binsize=n, threshold=snr)["index"][:, 0, 0]
+        detected += int((a >= 0).sum())
+        omitted += int(((a >= 0) & (b < 0)).sum())
+    assert detected > 1000
+    rate = omitted / detected
+    print("fd budget %.3g: omitted %d of %d detections (%.4f)" % (fd, omitted, detected, rate))
+    assert rate <= 2 * fd, rate
+
+
+def test_sub_ranges_match_slices(gpu_device):
+    n = 4096
+    power, D, H = loud_case(n, 6, 5, seed=9)
+    hier = mf.HierarchicalFilter(n, 6, 5, band=512, device="gpu")
+    hier.set_reference(power)
+    hier.set_coarse_threshold(3.0)
+    hier.set_data(D)
+    hier.set_templates(H)
+    full = hier.run(binsize=512, threshold=5.0).copy()
+    sub = hier.run(binsize=512, threshold=5.0, data=(2, 3), templates=(1, 3))
+    np.testing.assert_array_equal(sub["index"], full["index"][2:5, 1:4])
+    np.testing.assert_array_equal(sub["value"], full["value"][2:5, 1:4])

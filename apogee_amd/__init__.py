@@ -263,6 +263,65 @@ class MatchedFilter:
         return self._gpu.peaks(d0, nd, t0, nt, binsize, float(threshold), start, end,
                                counts=bool(counts))
 
+    # ------------------------------------------------------------ series
+    def _series_layout(self, series, starts, win_start, win_end, binsize, templates):
+        """Validate the run_series contract (python/matchedfilter/__init__.py:517-543)."""
+        ser = _from_any(series)
+        if _is_device_tensor(ser):
+            import torch
+            if ser.dim() != 1:
+                raise ValueError("series, starts, win_start and win_end must be one-dimensional")
+            ser = ser.to(torch.complex64).contiguous()
+        else:
+            ser = np.ascontiguousarray(ser, dtype=np.complex64)
+            if ser.ndim != 1:
+                raise ValueError("series, starts, win_start and win_end must be one-dimensional")
+        arrs = [np.asarray(_from_any(a)) for a in (starts, win_start, win_end)]
+        if any(a.ndim != 1 for a in arrs):
+            raise ValueError("series, starts, win_start and win_end must be one-dimensional")
+        if any(np.issubdtype(a.dtype, np.signedinteger) and (a < 0).any() for a in arrs):
+            raise ValueError("starts and windows must be nonnegative")
+        st, ws, we = (np.ascontiguousarray(a, dtype=np.uintp) for a in arrs)
+        if not (st.size == ws.size == we.size):
+            raise ValueError("starts, win_start and win_end must be the same length")
+        if st.size < 1:
+            raise ValueError("run_series needs at least one block")
+        if np.any(st > np.iinfo(np.intp).max - self.n):
+            raise ValueError("starts must be nonnegative and fit in the sample index range")
+        binsize = self.n if binsize is None else int(binsize)
+        if binsize < 1:
+            raise ValueError("binsize must be >= 1")
+        t0, nt = (0, self.ntemplates) if templates is None else (int(templates[0]), int(templates[1]))
+        if nt < 1 or t0 < 0 or t0 + nt > self.ntemplates:
+            raise ValueError("templates sub-range out of bounds")
+        self._require_templates(t0, nt)
+        from ._series import SeriesLayout
+        layout = SeriesLayout(self.n, st, ws, we, binsize)
+        return ser, layout, binsize, t0, nt
+
+    def run_series(self, series, starts, win_start, win_end,
+                   binsize=None, threshold=0.0, templates=None, raw=False):
+        """Filter a time series over a caller-supplied block layout.
+
+        Block b covers ``series[starts[b]:starts[b]+n]`` (zero-padded; its
+        spectrum is FFT(block)/n) and reports lags [win_start[b],
+        win_end[b]).  Every window must give the same number of bins.  The
+        forward transforms and the filter run on the device.  Returns
+        (nblocks, ntemplates, nbins) in caller order.  A later ``run()``
+        requires ``set_data()`` again (python/matchedfilter/__init__.py:545-594).
+        """
+        ser, layout, binsize, t0, nt = self._series_layout(series, starts, win_start, win_end,
+                                                           binsize, templates)
+        self._dataset = False
+        self._data_ready = set()
+        idx, val = self._dispatch_series(ser, layout, binsize, float(threshold), t0, nt)
+        return _format_result(idx, val, raw=raw)
+
+    def _dispatch_series(self, ser, layout, binsize, threshold, t0, nt):
+        idx, val, _ = self._gpu.run_series(ser, layout.starts, layout.low, layout.high, t0, nt,
+                                           binsize, threshold)
+        return idx, val
+
     # ---------------------------------------------------------- memory
     def empty_shared(self, shape, dtype=np.complex64):
         """Host array suitable for the setters (ordinary NumPy storage)."""

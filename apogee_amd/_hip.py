@@ -58,6 +58,13 @@ def _declare(lib):
     lib.mfh_hier_peaks_async.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _c_size, ctypes.c_float, _c_size, _c_size, ctypes.c_float,
                                          _vp, _vp, _vp]
+    for name in ("mfh_run_series", "mfh_hier_run_series"):
+        getattr(lib, name).restype = ctypes.c_long
+    lib.mfh_run_series.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, _c_size, ctypes.c_float, _vp, _vp, _vp]
+    lib.mfh_hier_run_series.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, _c_size, ctypes.c_float,
+                                        ctypes.c_float, _vp, _vp, _vp, _vp]
     lib.mfh_set_tuning.argtypes = [_vp, ctypes.c_char_p, ctypes.c_long]
     lib.mfh_prof_enable.argtypes = [_vp, ctypes.c_int]
     lib.mfh_prof_read.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
@@ -250,6 +257,38 @@ class Context:
                                          float(threshold), int(start), int(end), float(coarse_threshold),
                                          ctypes.c_void_p(int(idx_ptr)), ctypes.c_void_p(int(val_ptr)), None),
               "mfh_hier_peaks_async")
+
+    # ------------------------------------------------------------ series
+    def run_series(self, series, starts, win_start, win_end, t0, nt, binsize, threshold,
+                   coarse_threshold=None):
+        """Block spectra of a series on the device, then the flat (or, with a
+        coarse threshold, hierarchical) filter; returns (index, value, refined)."""
+        dev = _device_pointer(series)
+        if dev is not None:
+            import torch
+            torch.cuda.current_stream(series.device).synchronize()
+            sp, ns, on_dev = ctypes.c_void_p(dev), int(series.numel()), 1
+        else:
+            sp, ns, on_dev = _ptr(series), int(series.size), 0
+        st = np.ascontiguousarray(starts, dtype=np.uintp)
+        ws = np.ascontiguousarray(win_start, dtype=np.uintp)
+        we = np.ascontiguousarray(win_end, dtype=np.uintp)
+        nblk = int(st.size)
+        nb = int(lib().mfh_nbins(self.n, int(binsize), int(ws[0]), int(we[0])))
+        idx = np.empty((nblk, nt, nb), dtype=np.int64)
+        val = np.empty((nblk, nt, nb), dtype=np.complex64)
+        refined = ctypes.c_int64(0)
+        if coarse_threshold is None:
+            check(lib().mfh_run_series(self.handle, sp, ns, on_dev, _ptr(st), _ptr(ws), _ptr(we), nblk,
+                                       int(t0), int(nt), int(binsize), float(threshold),
+                                       _ptr(idx), _ptr(val), None), "mfh_run_series")
+        else:
+            check(lib().mfh_hier_run_series(self.handle, sp, ns, on_dev, _ptr(st), _ptr(ws), _ptr(we),
+                                            nblk, int(t0), int(nt), int(binsize), float(threshold),
+                                            float(coarse_threshold), _ptr(idx), _ptr(val), None,
+                                            ctypes.byref(refined)), "mfh_hier_run_series")
+        self.last_refinements = int(refined.value)
+        return idx, val, int(refined.value)
 
     # ------------------------------------------------------------- tuning
     def set_tuning(self, key, value):

@@ -102,12 +102,12 @@ int hier_prepare(Ctx *c) {
 }
 
 int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,
-             float coarse_thr, int64_t *idx, cf *val, int *count_out) {
+             float coarse_thr, int64_t *idx, cf *val, int *count_out, const cf *data_rows) {
   if (c->coarse_band <= 0) return fail("no coarse band: call mfh_hier_set_band first");
   if (!(coarse_thr >= 0.f) || !std::isfinite(coarse_thr))
     return fail("coarse threshold must be finite and nonnegative");
-  if (d0 < 0 || t0 < 0 || nd < 1 || nt < 1 || d0 + nd > c->nd || t0 + nt > c->nt)
-    return fail("data/templates sub-range out of bounds");
+  if (!data_rows && (d0 < 0 || d0 + nd > c->nd)) return fail("data/templates sub-range out of bounds");
+  if (t0 < 0 || nd < 1 || nt < 1 || t0 + nt > c->nt) return fail("data/templates sub-range out of bounds");
   if (binsize < 1) return fail("binsize must be >= 1");
   if (we > c->n) we = c->n;
   if (ws >= we) return fail("empty window (%zu, %zu)", ws, we);
@@ -127,7 +127,8 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   if (!cidx || !cval || !list || !count) return -1;
 
   FlatRun cr;
-  cr.data = c->data + (size_t)d0 * n;
+  const cf *drows = data_rows ? data_rows : c->data + (size_t)d0 * n;
+  cr.data = drows;
   cr.tmpl = c->coarse_tmpl + (size_t)t0 * m;
   cr.data_stride = (long long)n;
   cr.tmpl_stride = (long long)m;
@@ -151,7 +152,7 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   if (!hip_ok(hipGetLastError(), "gate_kernel")) return -1;
 
   FlatRun rr;
-  rr.data = c->data + (size_t)d0 * n;
+  rr.data = drows;
   rr.tmpl = c->tmpl + (size_t)t0 * n;
   rr.data_stride = (long long)n;
   rr.tmpl_stride = (long long)n;
@@ -197,7 +198,7 @@ extern "C" int mfh_hier_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt
   if (!ctx) return fail("null context");
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   return mfh::hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, dev_idx, (mfh::cf *)dev_val_c64,
-                       host_refined);
+                       host_refined, nullptr);
 }
 
 extern "C" long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
@@ -216,7 +217,8 @@ extern "C" long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, siz
   mfh::cf *dval = (mfh::cf *)c->scratch("out_val", cells * sizeof(mfh::cf));
   if (!didx || !dval) return -1;
   int refined = 0;
-  if (mfh::hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, didx, dval, &refined) < 0) return -1;
+  if (mfh::hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, didx, dval, &refined, nullptr) < 0)
+    return -1;
   if (!hip_ok(hipMemcpyAsync(idx, didx, cells * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream), "copy idx") ||
       !hip_ok(hipMemcpyAsync(val_c64, dval, cells * sizeof(mfh::cf), hipMemcpyDeviceToHost, c->stream),
               "copy val") ||

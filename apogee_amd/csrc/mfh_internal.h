@@ -87,5 +87,11 @@ int flat_fill_twiddles(Ctx *c, size_t n, cf *tw);
 // Validates, splits windows whose bin table would not fit LDS, and launches
 // the length-n flat kernel on c->stream.  Returns 0 or -1 (message set).
 int launch_flat(Ctx *c, size_t n, const FlatRun &r);
+// Forward spectra of run_series blocks into out [nblocks][n] (device).
+int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out);
+// Hierarchical run over data rows [d0, d0+nd) of the context, or over
+// `data_rows` (stride n) when not null.
+int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,
+             float coarse_thr, int64_t *idx, cf *val, int *count_out, const cf *data_rows);
 
 }  // namespace mfh

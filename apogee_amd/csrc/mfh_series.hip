@@ -1,0 +1,216 @@
+// run_series on the device: gather each block of a time series, zero-pad,
+// scale by 1/n, forward FFT, and filter the block spectra -- no spectrum
+// crosses the bus.
+//
+// Reference behaviour (paths relative to the reference checkout):
+//   block spectrum FFT(series[s:s+n])/n, zero-padded,
+//   1/n applied before the transform (exact for n = 2^k)   src/matchfilt.c:437-490
+//   equal-window runs filtered together, equal bin counts   src/matchfilt.c:449-488
+//   GPU precedent (seriesForward via conj(IFFT(conj x)))    src/gpu/series_forward.slang:9-33
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "mfh_flat.h"
+#include "mfh_internal.h"
+#include "mfhip.h"
+
+namespace mfh {
+
+namespace {
+
+// One transform slot per block: thread t loads x[t + u*TPT] (conjugated and
+// scaled by 1/N), runs the inverse FFT and writes conj(z) -- the forward
+// transform -- in natural order.
+template <int N>
+__global__ void __launch_bounds__(FlatShape<N>::WG) __attribute__((amdgpu_waves_per_eu(4)))
+series_forward_kernel(const cf *series, long long nseries, const long long *starts, int nblocks, const cf *tw,
+                      cf *out) {
+  using S = FlatShape<N>;
+  constexpr int TPT = S::TPT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cf *xch_all = reinterpret_cast<cf *>(smem);
+  const int slot = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int b = blockIdx.x * S::SLOTS + slot;
+  const bool valid = b < nblocks;
+  const long long s0 = valid ? starts[b] : 0;
+  const float inv = 1.0f / (float)N;
+  cf v[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const long long k = s0 + t + u * TPT;
+    const cf x = (valid && k < nseries) ? series[k] : cf{0.f, 0.f};
+    v[u] = cf{x.x * inv, -(x.y * inv)};
+  }
+  inverse_fft<N>(v, t, xch_all + slot * S::XCH, tw);
+  if (valid) {
+    cf *o = out + (long long)b * N;
+#pragma unroll
+    for (int u = 0; u < 16; u++) o[t + u * TPT] = cf{v[u].x, -v[u].y};
+  }
+}
+
+template <int N>
+int launch_fwd(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out) {
+  using S = FlatShape<N>;
+  const cf *tw = c->twiddles(N);
+  if (!tw) return -1;
+  const size_t lds = (size_t)S::SLOTS * S::XCH * sizeof(cf);
+  static bool lds_set = false;
+  if (!lds_set) {
+    if (!hip_ok(hipFuncSetAttribute((const void *)series_forward_kernel<N>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                "hipFuncSetAttribute"))
+      return -1;
+    lds_set = true;
+  }
+  Ctx::ProfScope ps(c, "series_forward");
+  hipLaunchKernelGGL(series_forward_kernel<N>, dim3((unsigned)((nblocks + S::SLOTS - 1) / S::SLOTS)), dim3(S::WG),
+                     lds, c->stream, series, (long long)nseries, starts, nblocks, tw, out);
+  return hip_ok(hipGetLastError(), "series_forward_kernel") ? 0 : -1;
+}
+
+}  // namespace
+
+int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out) {
+  switch (c->n) {
+    case 64: return launch_fwd<64>(c, series, nseries, starts, nblocks, out);
+    case 128: return launch_fwd<128>(c, series, nseries, starts, nblocks, out);
+    case 256: return launch_fwd<256>(c, series, nseries, starts, nblocks, out);
+    case 512: return launch_fwd<512>(c, series, nseries, starts, nblocks, out);
+    case 1024: return launch_fwd<1024>(c, series, nseries, starts, nblocks, out);
+    case 2048: return launch_fwd<2048>(c, series, nseries, starts, nblocks, out);
+    case 4096: return launch_fwd<4096>(c, series, nseries, starts, nblocks, out);
+    case 8192: return launch_fwd<8192>(c, series, nseries, starts, nblocks, out);
+    case 16384: return launch_fwd<16384>(c, series, nseries, starts, nblocks, out);
+    default: return fail("transform length %zu is not supported", c->n);
+  }
+}
+
+}  // namespace mfh
+
+using mfh::cf;
+using mfh::Ctx;
+using mfh::fail;
+using mfh::hip_ok;
+
+namespace {
+
+// Shared body of the flat and hierarchical run_series entry points.
+long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int series_on_device, const size_t *start,
+                const size_t *win_start, const size_t *win_end, int nblocks, int t0, int nt, size_t binsize,
+                float threshold, float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
+                int64_t *n_refined) {
+  if (nblocks < 1) return fail("run_series needs at least one block");
+  if (!series_c64 || !start || !win_start || !win_end || !idx || !val_c64) return fail("null argument");
+  if (t0 < 0 || nt < 1 || t0 + nt > c->nt) return fail("templates sub-range out of bounds");
+  if (binsize < 1) return fail("binsize must be >= 1");
+  const size_t n = c->n;
+  const size_t nb = mfh_nbins(n, binsize, win_start[0], win_end[0]);
+  if (nb == 0) return fail("every block must have a nonempty search window");
+  for (int b = 0; b < nblocks; b++) {
+    const size_t bb = mfh_nbins(n, binsize, win_start[b], win_end[b]);
+    if (bb == 0) return fail("every block must have a nonempty search window");
+    if (bb != nb) return fail("every block's window must give the same bin count");
+  }
+  hipSetDevice(c->dev);
+  cf *spec = (cf *)c->scratch("series_spec", (size_t)nblocks * n * sizeof(cf));
+  long long *dstart = (long long *)c->scratch("series_starts", (size_t)nblocks * sizeof(long long));
+  const size_t cells = (size_t)nblocks * nt * nb;
+  int64_t *didx = (int64_t *)c->scratch("out_idx", cells * sizeof(int64_t));
+  cf *dval = (cf *)c->scratch("out_val", cells * sizeof(cf));
+  if (!spec || !dstart || !didx || !dval) return -1;
+  const cf *dser;
+  if (series_on_device) {
+    dser = (const cf *)series_c64;
+  } else {
+    cf *buf = (cf *)c->scratch("series_in", (nseries ? nseries : 1) * sizeof(cf));
+    if (!buf) return -1;
+    if (nseries && !hip_ok(hipMemcpyAsync(buf, series_c64, nseries * sizeof(cf), hipMemcpyHostToDevice, c->stream),
+                           "copy series"))
+      return -1;
+    dser = buf;
+  }
+  std::vector<long long> st(nblocks);
+  for (int b = 0; b < nblocks; b++) st[b] = start[b] < nseries ? (long long)start[b] : (long long)nseries;
+  if (!hip_ok(hipMemcpyAsync(dstart, st.data(), nblocks * sizeof(long long), hipMemcpyHostToDevice, c->stream),
+              "copy starts") ||
+      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+    return -1;
+  if (mfh::launch_series_forward(c, dser, nseries, dstart, nblocks, spec) < 0) return -1;
+  long long refined_total = 0;
+  std::vector<int> refined;
+  // consecutive blocks with equal windows share one launch
+  for (int b0 = 0; b0 < nblocks;) {
+    int g = 1;
+    while (b0 + g < nblocks && win_start[b0 + g] == win_start[b0] && win_end[b0 + g] == win_end[b0]) g++;
+    const size_t ws = win_start[b0], we = std::min(win_end[b0], n);
+    int64_t *oi = didx + (size_t)b0 * nt * nb;
+    cf *ov = dval + (size_t)b0 * nt * nb;
+    if (hier) {
+      refined.push_back(0);
+      if (mfh::hier_run(c, 0, g, t0, nt, binsize, threshold, ws, we, coarse_thr, oi, ov, nullptr,
+                        spec + (size_t)b0 * n) < 0)
+        return -1;
+      // survivor counts are read from the device list counter per group
+      int cnt = 0;
+      int *dcount = (int *)c->scratch("refine_count", sizeof(int));
+      if (!hip_ok(hipMemcpyAsync(&cnt, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy count") ||
+          !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+        return -1;
+      refined_total += cnt;
+    } else {
+      mfh::FlatRun r;
+      r.data = spec + (size_t)b0 * n;
+      r.tmpl = c->tmpl + (size_t)t0 * n;
+      r.data_stride = (long long)n;
+      r.tmpl_stride = (long long)n;
+      r.nd = g;
+      r.nt = nt;
+      r.binsize = binsize;
+      r.threshold = threshold;
+      r.ws = ws;
+      r.we = we;
+      r.out_idx = oi;
+      r.out_val = ov;
+      if (mfh::launch_flat(c, n, r) < 0) return -1;
+    }
+    b0 += g;
+  }
+  if (!hip_ok(hipMemcpyAsync(idx, didx, cells * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream), "copy idx") ||
+      !hip_ok(hipMemcpyAsync(val_c64, dval, cells * sizeof(cf), hipMemcpyDeviceToHost, c->stream), "copy val") ||
+      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+    return -1;
+  c->prof_collect();
+  if (n_refined) *n_refined = refined_total;
+  long total = 0;
+  for (size_t p = 0; p < (size_t)nblocks * nt; p++) {
+    int cnt = 0;
+    for (size_t b = 0; b < nb; b++) cnt += idx[p * nb + b] >= 0;
+    if (counts) counts[p] = cnt;
+    total += cnt;
+  }
+  return total;
+}
+
+}  // namespace
+
+extern "C" long mfh_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, int series_on_device,
+                               const size_t *start, const size_t *win_start, const size_t *win_end, int nblocks,
+                               int t0, int nt, size_t binsize, float threshold, int64_t *idx, void *val_c64,
+                               int32_t *counts) {
+  if (!ctx) return fail("null context");
+  return run_series(reinterpret_cast<Ctx *>(ctx), false, series_c64, nseries, series_on_device, start, win_start,
+                    win_end, nblocks, t0, nt, binsize, threshold, 0.f, idx, val_c64, counts, nullptr);
+}
+
+extern "C" long mfh_hier_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, int series_on_device,
+                                    const size_t *start, const size_t *win_start, const size_t *win_end,
+                                    int nblocks, int t0, int nt, size_t binsize, float threshold, float coarse_thr,
+                                    int64_t *idx, void *val_c64, int32_t *counts, int64_t *n_refined) {
+  if (!ctx) return fail("null context");
+  return run_series(reinterpret_cast<Ctx *>(ctx), true, series_c64, nseries, series_on_device, start, win_start,
+                    win_end, nblocks, t0, nt, binsize, threshold, coarse_thr, idx, val_c64, counts, n_refined);
+}

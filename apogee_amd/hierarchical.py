@@ -143,6 +143,17 @@ class HierarchicalFilter(MatchedFilter):
         self._trig += int(refined)
         return idx, val, cnt
 
+    def _dispatch_series(self, ser, layout, binsize, threshold, t0, nt):
+        band, f, thr = self._calibration()
+        if self._band_set != (band, f):
+            self._gpu.hier_set_band(band, f)
+            self._band_set = (band, f)
+        idx, val, refined = self._gpu.run_series(ser, layout.starts, layout.low, layout.high, t0, nt,
+                                                 binsize, threshold, coarse_threshold=thr)
+        self._pairs += idx.shape[0] * idx.shape[1]
+        self._trig += int(refined)
+        return idx, val
+
     def _on_ingest(self, kind, index, a):
         if kind == "template":
             self._band_set = None

@@ -142,6 +142,37 @@ int mfh_hier_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
                          float coarse_thr, int64_t *dev_idx, void *dev_val_c64,
                          int32_t *host_refined);
 
+/* ---- time series --------------------------------------------------------- */
+
+/* Filter a time series over a caller-supplied block layout: block b covers
+ * series[start[b] .. start[b]+n) (zero-padded past the end; spectrum
+ * FFT(block)/n with the 1/n applied first) and reports lags
+ * [win_start[b], win_end[b]) in bins of `binsize`.  Every window must give
+ * the same bin count.  `series_c64` is nseries complex64 samples in host
+ * memory, or a device pointer when series_on_device is non-zero.  Outputs are
+ * [block][template][bin] in the caller's block order; counts (optional) per
+ * (block, template).  Runs of consecutive equal windows share one launch.
+ * Returns total crossings, or < 0.
+ * Replaces: ap_mf_run_series (python/matchedfilter/matchedfilter.h:84-96;
+ * src/matchfilt.c:437-490) and Context.forward + peaks of the reference's
+ * GPU run_series (python/matchedfilter/_vkcompute.py:855-906,
+ * python/matchedfilter/__init__.py:601-645). */
+long mfh_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, int series_on_device,
+                    const size_t *start, const size_t *win_start, const size_t *win_end, int nblocks,
+                    int t0, int nt, size_t binsize, float threshold,
+                    int64_t *idx, void *val_c64, int32_t *counts);
+
+/* The hierarchical twin: the block spectra go through the coarse screen and
+ * refine of mfh_hier_peaks (band set by mfh_hier_set_band); *n_refined
+ * receives the number of refined (block, template) pairs.
+ * Replaces: ap_hmf_run_series (python/matchedfilter/matchedfilter.h:186-191;
+ * src/hmf.c:158-220). */
+long mfh_hier_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, int series_on_device,
+                         const size_t *start, const size_t *win_start, const size_t *win_end,
+                         int nblocks, int t0, int nt, size_t binsize, float threshold,
+                         float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
+                         int64_t *n_refined);
+
 /* Tuning knobs: key "dpw" (data rows per workgroup; 0 = automatic).
  * Replaces: MF_MFTILE / MF_PBMAX environment knobs (src/matchfilt.c:93-107). */
 int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value);

@@ -54,3 +54,11 @@ class FakeContext:
         cnt = (idx >= 0).sum(axis=2).astype(np.int32) if counts else None
         self.last_refinements = int(fired.sum())
         return idx, val.astype(np.complex64), cnt, int(fired.sum())
+
+    def run_series(self, series, starts, win_start, win_end, t0, nt, binsize, threshold,
+                   coarse_threshold=None):
+        FakeContext.calls.append(("series", len(starts), t0, nt, binsize, threshold, coarse_threshold))
+        T = self.T[t0:t0 + nt]
+        idx, val, _ = O.series_peaks(np.asarray(series), starts, win_start, win_end, T, binsize,
+                                     threshold)
+        return idx, val.astype(np.complex64), 0

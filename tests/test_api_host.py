@@ -210,3 +210,35 @@ def test_gate_model_invariants():
     flat = np.zeros(4096)
     flat[:512] = 1.0
     assert G.gate_for(flat, 4096, 512, 5.5, 1e-2) < g2
+
+
+# --------------------------------------------------------------- series
+
+def test_run_series_contract(fake):
+    rng = np.random.default_rng(6)
+    n = 256
+    f = mf.MatchedFilter(n, 1, 2)
+    f.set_templates(cnoise(rng, (2, n)))
+    s = cnoise(rng, 2000)
+    with pytest.raises(ValueError):
+        f.run_series(s, [0, 100], [0], [10])                       # lengths differ
+    with pytest.raises(ValueError):
+        f.run_series(s, [], [], [])                                # no blocks
+    with pytest.raises(ValueError):
+        f.run_series(s, [0, 100], [0, 0], [256, 100], binsize=64)  # unequal bin counts
+    with pytest.raises(ValueError):
+        f.run_series(s, [0], [50], [50])                           # empty window
+    with pytest.raises(ValueError):
+        f.run_series(s.reshape(2, 1000), [0], [0], [10])           # not 1-D
+    with pytest.raises(ValueError):
+        f.run_series(s, [-1], [0], [10])                           # negative start
+    with pytest.raises(ValueError):
+        f.run_series(s, [0], [0], [10], templates=(1, 5))
+    out = f.run_series(s.real.astype(np.float32), [0, 300, 1900], [0, 0, 0], [256, 256, 256],
+                       binsize=64)
+    assert out.shape == (3, 2, 4)
+    want = O.series_peaks(s.real.astype(np.complex64), [0, 300, 1900], [0, 0, 0], [256, 256, 256],
+                          f._gpu.T, 64, 0.0)
+    np.testing.assert_array_equal(out["index"], want[0])
+    with pytest.raises(ValueError, match="no data"):
+        f.run()                                                    # series consumed the slots

@@ -1,0 +1,155 @@
+"""run_series on the MI355X: device block spectra + filter against the FP64
+oracle (block spectrum FFT(series[s:s+n])/n, zero-padded, then the flat
+filter with the block's window), the hierarchical one-sided guarantee, and
+the reference's contract (tests/test_run_series_flat.py, tests/test_api.py:376-476).
+"""
+import numpy as np
+import pytest
+
+import apogee_amd as mf
+from oracle import mf_oracle as O
+from _parity import assert_peaks_match
+
+pytestmark = pytest.mark.gpu
+
+
+def overlap_save_layout(nseries, n, ntaps):
+    """Overlap-save blocks stepping n - ntaps + 1; the last block is ragged."""
+    valid = n - ntaps + 1
+    starts, ws, we = [], [], []
+    s = 0
+    while s < nseries:
+        starts.append(s)
+        ws.append(ntaps - 1)
+        we.append(min(n, ntaps - 1 + (nseries - s)))
+        s += valid
+    return (np.array(starts, np.uintp), np.array(ws, np.uintp), np.array(we, np.uintp))
+
+
+def check_against_oracle(series, starts, ws, we, T, binsize, threshold, got):
+    n = T.shape[1]
+    spec = O.series_spectra(series, starts, n)
+    for b in range(len(starts)):
+        assert_peaks_match(spec[b:b + 1], T, got["index"][b:b + 1], got["value"][b:b + 1], binsize,
+                           threshold, (int(ws[b]), int(we[b])), context="block %d" % b)
+
+
+@pytest.mark.parametrize("n", [256, 1024, 4096])
+def test_half_overlap_real_series(n, gpu_device):
+    """A real float32 series with 50% overlap (starts = b n/2), window [n/4, 3n/4)."""
+    rng = np.random.default_rng(n)
+    nt, nblk = 5, 12
+    series = rng.standard_normal(nblk * n // 2 + n // 2).astype(np.float32)
+    T = (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64)
+    starts = (np.arange(nblk) * (n // 2)).astype(np.uintp)
+    ws = np.full(nblk, n // 4, np.uintp)
+    we = np.full(nblk, 3 * n // 4, np.uintp)
+    f = mf.MatchedFilter(n, 1, nt, device="gpu")
+    f.set_templates(T)
+    got = f.run_series(series, starts, ws, we, binsize=n // 8, threshold=0.0)
+    assert got.shape == (nblk, nt, 4)
+    check_against_oracle(series.astype(np.complex64), starts, ws, we, T, n // 8, 0.0, got)
+
+
+def test_ragged_edges_and_zero_padding(gpu_device):
+    n, nt = 2048, 3
+    rng = np.random.default_rng(2)
+    series = (rng.standard_normal(9000) + 1j * rng.standard_normal(9000)).astype(np.complex64)
+    T = (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64)
+    starts, ws, we = overlap_save_layout(series.size, n, 451)
+    nb_first = -(-(int(we[0]) - int(ws[0])) // n)
+    f = mf.MatchedFilter(n, 1, nt, device="gpu")
+    f.set_templates(T)
+    got = f.run_series(series, starts, ws, we, binsize=n, threshold=0.0)
+    assert got.shape == (len(starts), nt, nb_first)
+    check_against_oracle(series, starts, ws, we, T, n, 0.0, got)
+
+
+def test_blocks_past_the_end_and_caller_order(gpu_device):
+    n, nt = 1024, 2
+    rng = np.random.default_rng(3)
+    series = (rng.standard_normal(3000) + 1j * rng.standard_normal(3000)).astype(np.complex64)
+    T = (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64)
+    starts = np.array([2500, 0, 5000, 1200, 0], np.uintp)          # one block entirely past the end
+    ws = np.array([0, 100, 0, 100, 0], np.uintp)
+    we = np.array([512, 612, 512, 612, 512], np.uintp)
+    f = mf.MatchedFilter(n, 1, nt, device="gpu")
+    f.set_templates(T)
+    got = f.run_series(series, starts, ws, we, binsize=128, threshold=0.0)
+    check_against_oracle(series, starts, ws, we, T, 128, 0.0, got)
+    assert (got["index"][2] == -1).all()                          # an all-zero block reports nothing
+    idx, val = f.run_series(series, starts, ws, we, binsize=128, threshold=0.0, raw=True)
+    np.testing.assert_array_equal(idx, got["index"])
+    with pytest.raises(ValueError, match="no data"):
+        f.run()                                                    # data slots consumed
+
+
+def test_series_matches_block_by_block_runs(gpu_device):
+    """run_series == set_data(FFT(block)/n) + run(window), to FP32 rounding."""
+    n, nt = 4096, 4
+    rng = np.random.default_rng(21)
+    series = rng.standard_normal(1 << 15).astype(np.float32)
+    T = (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64)
+    starts, ws, we = overlap_save_layout(series.size, n, 451)
+    keep = [i for i in range(len(starts)) if we[i] - ws[i] == we[0] - ws[0]]
+    starts, ws, we = starts[keep], ws[keep], we[keep]
+    f = mf.MatchedFilter(n, 1, nt, device="gpu")
+    f.set_templates(T)
+    got = f.run_series(series, starts, ws, we, binsize=n, threshold=0.0).copy()
+    for b, s in enumerate(starts):
+        blk = np.zeros(n, np.complex64)
+        seg = series[int(s):int(s) + n]
+        blk[:seg.size] = seg
+        f.set_data((np.fft.fft(blk) / n).astype(np.complex64)[None, :])
+        want = f.run(binsize=n, threshold=0.0, window=(int(ws[b]), int(we[b])))
+        same = got["index"][b] == want["index"][0]
+        assert same.mean() >= 0.75          # FP32 forward transforms differ only at near-ties
+        np.testing.assert_allclose(np.abs(got["value"][b][same]), np.abs(want["value"][0][same]),
+                                   rtol=2e-6 * 12)
+
+
+def test_hierarchical_series_is_identical_or_absent(gpu_device):
+    n, nt = 4096, 6
+    rng = np.random.default_rng(11)
+    k = np.arange(1, n // 2)
+    power = np.zeros(n, np.float32)
+    power[1:n // 2] = k ** (-7 / 3.0) / ((0.015 * n / k) ** 4 + 1.0)
+    power /= power.sum()
+    H = np.stack([(np.sqrt(power) * np.exp(2j * np.pi * rng.random(n))) for _ in range(nt)])
+    H = (H / np.linalg.norm(H, axis=1, keepdims=True)).astype(np.complex64)
+    series = (rng.standard_normal(1 << 16) + 1j * rng.standard_normal(1 << 16)).astype(np.complex64)
+    # plant a loud template copy in the series
+    h_time = np.fft.ifft(H[2].astype(np.complex128)) * n
+    series[20000:20000 + n] += (12 * n * h_time / n).astype(np.complex64)
+    starts, ws, we = overlap_save_layout(series.size, n, 451)
+    flat = mf.MatchedFilter(n, 1, nt, device="gpu")
+    hier = mf.HierarchicalFilter(n, 1, nt, snr=5.0, fd=1e-2, device="gpu")
+    hier.set_reference(power)
+    flat.set_templates(H)
+    hier.set_templates(H)
+    a = flat.run_series(series, starts, ws, we, binsize=512, threshold=5.0)
+    b = hier.run_series(series, starts, ws, we, binsize=512, threshold=5.0)
+    fired = b["index"] >= 0
+    assert fired.any(), "the planted copy was dismissed"
+    assert not (fired & (a["index"] < 0)).any()
+    np.testing.assert_array_equal(a["index"][fired], b["index"][fired])
+    np.testing.assert_array_equal(a["value"][fired], b["value"][fired])
+    pairs, trig = hier.stats
+    assert pairs == len(starts) * nt and trig < pairs
+
+
+def test_device_resident_series(gpu_device):
+    import torch
+    n, nt = 1024, 2
+    rng = np.random.default_rng(5)
+    series = (rng.standard_normal(5000) + 1j * rng.standard_normal(5000)).astype(np.complex64)
+    T = (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64)
+    starts = np.arange(0, 4000, 512).astype(np.uintp)
+    ws = np.zeros(len(starts), np.uintp)
+    we = np.full(len(starts), 512, np.uintp)
+    f = mf.MatchedFilter(n, 1, nt, device="gpu")
+    f.set_templates(T)
+    host = f.run_series(series, starts, ws, we, binsize=256).copy()
+    dev = f.run_series(torch.from_numpy(series).to("cuda"), starts, ws, we, binsize=256)
+    np.testing.assert_array_equal(host["index"], dev["index"])
+    np.testing.assert_array_equal(host["value"], dev["value"])

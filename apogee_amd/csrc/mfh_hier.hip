@@ -65,18 +65,29 @@ __global__ void zero_count_kernel(int *count) {
 // Gate one pair on its coarse peak: fire iff a peak was reported (|z|^2 above
 // thr^2) and sqrt(|z|^2) >= thr, exactly the CPU engine's two tests.  Fired
 // pairs are appended to the refine list; the rest get -1 / 0 in every bin.
+// One atomic per wave: lanes that fire take consecutive slots after the
+// wave's base, found from the ballot.
 __global__ void gate_kernel(const int64_t *cidx, const cf *cval, int nd, int nt, float thr, int nb,
                             int64_t *out_idx, cf *out_val, int2 *list, int *count) {
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= (long long)nd * nt) return;
+  const bool inside = p < (long long)nd * nt;
   bool fire = false;
-  if (cidx[p] >= 0) {
+  if (inside && cidx[p] >= 0) {
     const cf z = cval[p];
     const float m2 = __builtin_fmaf(z.x, z.x, z.y * z.y);
     fire = sqrtf(m2) >= thr;
   }
+  const unsigned long long ballot = __ballot(fire);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (ballot) {
+    const int leader = __ffsll((long long)ballot) - 1;
+    if (lane == leader) base = atomicAdd(count, __popcll(ballot));
+    base = __shfl(base, leader, 64);
+  }
+  if (!inside) return;
   if (fire) {
-    const int slot = atomicAdd(count, 1);
+    const int slot = base + __popcll(ballot & ((1ull << lane) - 1ull));
     list[slot] = int2{(int)(p / nt), (int)(p % nt)};
   } else {
     for (int b = 0; b < nb; b++) {

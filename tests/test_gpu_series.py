@@ -122,6 +122,8 @@ def test_hierarchical_series_is_identical_or_absent(gpu_device):
     h_time = np.fft.ifft(H[2].astype(np.complex128)) * n
     series[20000:20000 + n] += (12 * n * h_time / n).astype(np.complex64)
     starts, ws, we = overlap_save_layout(series.size, n, 451)
+    keep = [i for i in range(len(starts)) if we[i] - ws[i] == we[0] - ws[0]]
+    starts, ws, we = starts[keep], ws[keep], we[keep]
     flat = mf.MatchedFilter(n, 1, nt, device="gpu")
     hier = mf.HierarchicalFilter(n, 1, nt, snr=5.0, fd=1e-2, device="gpu")
     hier.set_reference(power)

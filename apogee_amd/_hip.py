@@ -65,6 +65,10 @@ def _declare(lib):
     lib.mfh_hier_run_series.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, _c_size, ctypes.c_float,
                                         ctypes.c_float, _vp, _vp, _vp, _vp]
+    lib.mfh_hier_set_coarse_precision.argtypes = [_vp, ctypes.c_int, ctypes.c_float]
+    lib.mfh_hier_coarse_max.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        _c_size, _c_size, ctypes.c_int, _vp]
+    lib.mfh_selftest_mfma.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
     lib.mfh_set_tuning.argtypes = [_vp, ctypes.c_char_p, ctypes.c_long]
     lib.mfh_prof_enable.argtypes = [_vp, ctypes.c_int]
     lib.mfh_prof_read.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
@@ -131,6 +135,16 @@ def device_info(index):
 
 def supported(n):
     return bool(lib().mfh_supported(int(n)))
+
+
+def selftest_mfma(device, variant, A, B):
+    """Run one MFMA variant on FP32 matrices (see include/mfhip.h)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    C = np.empty((16, 16), dtype=np.float32)
+    check(lib().mfh_selftest_mfma(int(device), int(variant), _ptr(A), _ptr(B), _ptr(C)),
+          "mfh_selftest_mfma")
+    return C
 
 
 def _ptr(a):
@@ -249,6 +263,18 @@ class Context:
                                    ctypes.byref(refined)), "mfh_hier_peaks")
         self.last_refinements = int(refined.value)
         return idx, val, cnt, int(refined.value)
+
+    def hier_set_coarse_precision(self, precision, delta):
+        """0 = FP32 screen, 1 = fp16 MFMA, 2 = bf16 MFMA with gate widening delta."""
+        check(lib().mfh_hier_set_coarse_precision(self.handle, int(precision), float(delta)),
+              "mfh_hier_set_coarse_precision")
+
+    def hier_coarse_max(self, d0, nd, t0, nt, start, end, precision):
+        """Coarse statistic max |z_c| per pair (nd, nt) at a precision."""
+        out = np.empty((nd, nt), dtype=np.float32)
+        check(lib().mfh_hier_coarse_max(self.handle, int(d0), int(nd), int(t0), int(nt), int(start),
+                                        int(end), int(precision), _ptr(out)), "mfh_hier_coarse_max")
+        return out
 
     def hier_peaks_async(self, d0, nd, t0, nt, binsize, threshold, start, end, coarse_threshold,
                          idx_ptr, val_ptr):

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "mfh_dft.h"
 #include "mfh_internal.h"
@@ -97,6 +98,35 @@ __global__ void gate_kernel(const int64_t *cidx, const cf *cval, int nd, int nt,
   }
 }
 
+// Gate on an MFMA coarse statistic: fire iff sqrt(cmax2) * (1 + delta) >= thr
+// (every pair when thr == 0), a superset of the FP32 screen's decisions
+// whenever the statistic's relative error is within delta.
+__global__ void gate_max_kernel(const float *cmax2, int nd, int nt, float thr, float delta, int nb,
+                                int64_t *out_idx, cf *out_val, int2 *list, int *count) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool inside = p < (long long)nd * nt;
+  bool fire = false;
+  if (inside) fire = thr == 0.f || sqrtf(cmax2[p]) * (1.0f + delta) >= thr;
+  const unsigned long long ballot = __ballot(fire);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (ballot) {
+    const int leader = __ffsll((long long)ballot) - 1;
+    if (lane == leader) base = atomicAdd(count, __popcll(ballot));
+    base = __shfl(base, leader, 64);
+  }
+  if (!inside) return;
+  if (fire) {
+    const int slot = base + __popcll(ballot & ((1ull << lane) - 1ull));
+    list[slot] = int2{(int)(p / nt), (int)(p % nt)};
+  } else {
+    for (int b = 0; b < nb; b++) {
+      out_idx[p * nb + b] = -1;
+      out_val[p * nb + b] = cf{0.f, 0.f};
+    }
+  }
+}
+
 }  // namespace
 
 int hier_prepare(Ctx *c) {
@@ -137,8 +167,17 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   int *count = (int *)c->scratch("refine_count", sizeof(int));
   if (!cidx || !cval || !list || !count) return -1;
 
-  FlatRun cr;
   const cf *drows = data_rows ? data_rows : c->data + (size_t)d0 * n;
+  const bool mfma = c->coarse_prec > 0 && m == 512;
+  float *cmax2 = nullptr;
+  if (mfma) {
+    cmax2 = (float *)c->scratch("coarse_max", pairs * sizeof(float));
+    if (!cmax2) return -1;
+    if (launch_coarse_mfma(c, drows, (long long)n, c->coarse_tmpl + (size_t)t0 * m, nd, nt, (int)cs, (int)ce,
+                           c->coarse_prec == 2, cmax2) < 0)
+      return -1;
+  }
+  FlatRun cr;
   cr.data = drows;
   cr.tmpl = c->coarse_tmpl + (size_t)t0 * m;
   cr.data_stride = (long long)n;
@@ -152,13 +191,17 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   cr.out_idx = cidx;
   cr.out_val = cval;
   cr.prof_name = "coarse";
-  if (launch_flat(c, m, cr) < 0) return -1;
+  if (!mfma && launch_flat(c, m, cr) < 0) return -1;
 
   hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(64), 0, c->stream, count);
   {
     Ctx::ProfScope ps(c, "gate");
-    hipLaunchKernelGGL(gate_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, c->stream, cidx, cval, nd,
-                       nt, coarse_thr, (int)nb, idx, val, list, count);
+    if (mfma)
+      hipLaunchKernelGGL(gate_max_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, c->stream, cmax2, nd,
+                         nt, coarse_thr, c->coarse_delta, (int)nb, idx, val, list, count);
+    else
+      hipLaunchKernelGGL(gate_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, c->stream, cidx, cval, nd,
+                         nt, coarse_thr, (int)nb, idx, val, list, count);
   }
   if (!hip_ok(hipGetLastError(), "gate_kernel")) return -1;
 
@@ -200,6 +243,70 @@ extern "C" int mfh_hier_set_band(mfh_ctx *ctx, size_t band, double ref_fraction)
   c->coarse_band = (int)band;
   c->coarse_ref_f = ref_fraction < 0.0 ? -1.0 : ref_fraction;
   c->coarse_dirty = true;
+  return 0;
+}
+
+extern "C" int mfh_hier_set_coarse_precision(mfh_ctx *ctx, int precision, float delta) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (precision < 0 || precision > 2) return fail("precision must be 0 (fp32), 1 (fp16) or 2 (bf16)");
+  if (!(delta >= 0.f) || delta >= 1.f) return fail("delta must be in [0, 1)");
+  c->coarse_prec = precision;
+  c->coarse_delta = delta;
+  return 0;
+}
+
+extern "C" int mfh_hier_coarse_max(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t ws, size_t we,
+                                   int precision, float *host_max) {
+  if (!ctx || !host_max) return fail("null argument");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (c->coarse_band <= 0) return fail("no coarse band: call mfh_hier_set_band first");
+  if (d0 < 0 || t0 < 0 || nd < 1 || nt < 1 || d0 + nd > c->nd || t0 + nt > c->nt)
+    return fail("data/templates sub-range out of bounds");
+  if (precision < 0 || precision > 2) return fail("precision must be 0 (fp32), 1 (fp16) or 2 (bf16)");
+  if (we > c->n) we = c->n;
+  if (ws >= we) return fail("empty window (%zu, %zu)", ws, we);
+  hipSetDevice(c->dev);
+  if (mfh::hier_prepare(c) < 0) return -1;
+  const size_t n = c->n, m = (size_t)c->coarse_band, pairs = (size_t)nd * nt;
+  const size_t R = n / m;
+  size_t cs = ws / R, ce = std::min(m, (we + R - 1) / R);
+  if (cs > 0) cs--;
+  float *cmax2 = (float *)c->scratch("coarse_max", pairs * sizeof(float));
+  if (!cmax2) return -1;
+  std::vector<float> m2(pairs);
+  if (precision > 0) {
+    if (mfh::launch_coarse_mfma(c, c->data + (size_t)d0 * n, (long long)n, c->coarse_tmpl + (size_t)t0 * m, nd, nt,
+                                (int)cs, (int)ce, precision == 2, cmax2) < 0 ||
+        !hip_ok(hipMemcpyAsync(m2.data(), cmax2, pairs * sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+      return -1;
+    for (size_t p = 0; p < pairs; p++) host_max[p] = sqrtf(m2[p]);
+    return 0;
+  }
+  int64_t *cidx = (int64_t *)c->scratch("coarse_idx", pairs * sizeof(int64_t));
+  mfh::cf *cval = (mfh::cf *)c->scratch("coarse_val", pairs * sizeof(mfh::cf));
+  if (!cidx || !cval) return -1;
+  mfh::FlatRun cr;
+  cr.data = c->data + (size_t)d0 * n;
+  cr.tmpl = c->coarse_tmpl + (size_t)t0 * m;
+  cr.data_stride = (long long)n;
+  cr.tmpl_stride = (long long)m;
+  cr.nd = nd;
+  cr.nt = nt;
+  cr.binsize = ce - cs;
+  cr.threshold = 0.f;
+  cr.ws = cs;
+  cr.we = ce;
+  cr.out_idx = cidx;
+  cr.out_val = cval;
+  cr.prof_name = "coarse";
+  std::vector<mfh::cf> v(pairs);
+  if (mfh::launch_flat(c, m, cr) < 0 ||
+      !hip_ok(hipMemcpyAsync(v.data(), cval, pairs * sizeof(mfh::cf), hipMemcpyDeviceToHost, c->stream), "copy") ||
+      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+    return -1;
+  for (size_t p = 0; p < pairs; p++) host_max[p] = sqrtf(__builtin_fmaf(v[p].x, v[p].x, v[p].y * v[p].y));
   return 0;
 }
 

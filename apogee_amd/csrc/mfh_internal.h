@@ -28,6 +28,11 @@ struct Ctx {
   double coarse_ref_f = -1.0;
   bool coarse_dirty = true;
   cf *coarse_tmpl = nullptr;
+  // coarse screen precision: 0 = FP32 flat screen, 1 = fp16 MFMA, 2 = bf16
+  // MFMA; the MFMA statistic is widened by (1 + coarse_delta) at the gate
+  int coarse_prec = 0;
+  float coarse_delta = 0.f;
+  bool coarse_roots_ready = false;
   int refine_grid_cap = 2048;  // workgroups of a list-driven (refine) launch
   // Per-pass twiddle tables of the flat kernel for length n (FP64-generated
   // on first use, kept for the context's lifetime).
@@ -89,6 +94,9 @@ int flat_fill_twiddles(Ctx *c, size_t n, cf *tw);
 int launch_flat(Ctx *c, size_t n, const FlatRun &r);
 // Forward spectra of run_series blocks into out [nblocks][n] (device).
 int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out);
+// MFMA coarse statistic (max |z_c|^2 over [cs, ce)) per pair, band 512.
+int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, const cf *ctmpl, int nd, int nt, int cs, int ce,
+                       bool bf16, float *cmax2);
 // Hierarchical run over data rows [d0, d0+nd) of the context, or over
 // `data_rows` (stride n) when not null.
 int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,

@@ -21,6 +21,13 @@ import numpy as np
 
 from . import MatchedFilter, _from_any
 
+#: Coarse-screen precisions: name -> (code of mfh_hier_set_coarse_precision,
+#: default gate widening).  The widening bounds the relative error of the
+#: matrix-core statistic against the FP32 screen with headroom: the largest
+#: error measured on noise, loud and chirp-bank pairs (tests/test_gpu_mfma.py)
+#: was ~1.5e-3 (fp16) and ~1.2e-2 (bf16); the defaults are 4x and 4x that.
+_COARSE_PRECISIONS = {"fp32": (0, 0.0), "fp16": (1, 6e-3), "bf16": (2, 5e-2)}
+
 #: Coarse band per transform length on gfx950 (power of two, 64 <= band < n).
 #: At n = 4096 the 512-point band is what the benchmark configuration pins.
 _GFX950_BAND = {1024: 128, 2048: 256, 4096: 512, 8192: 1024, 16384: 2048,
@@ -58,6 +65,8 @@ class HierarchicalFilter(MatchedFilter):
         self._band_set = None
         self._pairs = 0
         self._trig = 0
+        self._prec = ("fp32", 0.0)
+        self._prec_set = None
 
     # ----------------------------------------------------------- settings
     def set_reference(self, power):
@@ -93,6 +102,57 @@ class HierarchicalFilter(MatchedFilter):
         None or a non-positive value restores it."""
         self._fs_snr = float(snr) if snr is not None and float(snr) > 0 else None
         self._cal = None
+
+    def set_coarse_precision(self, precision="fp16", delta=None):
+        """Run the coarse screen on the matrix cores ("fp16" or "bf16", band
+        512) or in FP32 ("fp32", the default; any band).  The matrix-core
+        statistic is widened by ``1 + delta`` at the gate (default: the
+        measured bound of this build), so it keeps at least the pairs the FP32
+        screen keeps whenever its relative error is within ``delta``; reported
+        peaks are unaffected either way."""
+        if precision not in _COARSE_PRECISIONS:
+            raise ValueError("precision must be one of %s" % sorted(_COARSE_PRECISIONS))
+        d = _COARSE_PRECISIONS[precision][1] if delta is None else float(delta)
+        if not np.isfinite(d) or d < 0 or d >= 1:
+            raise ValueError("delta must be in [0, 1)")
+        self._prec = (precision, d)
+
+    @property
+    def coarse_precision(self):
+        """``(precision, delta)`` of the coarse screen."""
+        return self._prec
+
+    def coarse_statistic(self, precision=None, data=None, templates=None, window=None):
+        """The coarse maximum |z_c| per pair at a precision (default: the
+        configured one), shape (nd, nt); no refine runs.  Compare "fp16" or
+        "bf16" against "fp32" to measure the widening a bank needs."""
+        band, f, _ = self._calibration_band()
+        prec = self._prec[0] if precision is None else precision
+        if prec not in _COARSE_PRECISIONS:
+            raise ValueError("precision must be one of %s" % sorted(_COARSE_PRECISIONS))
+        start, end = self._window(window)
+        d0, nd = (0, self.ndata) if data is None else (int(data[0]), int(data[1]))
+        t0, nt = (0, self.ntemplates) if templates is None else (int(templates[0]), int(templates[1]))
+        self._apply_band(band, f)
+        return self._gpu.hier_coarse_max(d0, nd, t0, nt, start, end, _COARSE_PRECISIONS[prec][0])
+
+    def _calibration_band(self):
+        band = self._band()
+        f = -1.0
+        if self._ref is not None:
+            ref = self._ref.astype(np.float64)
+            f = float(ref[:band].sum() / ref.sum())
+        return band, f, None
+
+    def _apply_band(self, band, f):
+        if self._band_set != (band, f):
+            self._gpu.hier_set_band(band, f)
+            self._band_set = (band, f)
+        prec, delta = self._prec
+        code = _COARSE_PRECISIONS[prec][0] if band == 512 else 0
+        if self._prec_set != (code, delta):
+            self._gpu.hier_set_coarse_precision(code, delta)
+            self._prec_set = (code, delta)
 
     # ------------------------------------------------------- calibration
     def _band(self):
@@ -134,9 +194,7 @@ class HierarchicalFilter(MatchedFilter):
 
     def _dispatch(self, d0, nd, t0, nt, binsize, threshold, start, end, counts):
         band, f, thr = self._calibration()
-        if self._band_set != (band, f):
-            self._gpu.hier_set_band(band, f)
-            self._band_set = (band, f)
+        self._apply_band(band, f)
         idx, val, cnt, refined = self._gpu.hier_peaks(d0, nd, t0, nt, binsize, float(threshold),
                                                       start, end, thr, counts=bool(counts))
         self._pairs += nd * nt
@@ -145,9 +203,7 @@ class HierarchicalFilter(MatchedFilter):
 
     def _dispatch_series(self, ser, layout, binsize, threshold, t0, nt):
         band, f, thr = self._calibration()
-        if self._band_set != (band, f):
-            self._gpu.hier_set_band(band, f)
-            self._band_set = (band, f)
+        self._apply_band(band, f)
         idx, val, refined = self._gpu.run_series(ser, layout.starts, layout.low, layout.high, t0, nt,
                                                  binsize, threshold, coarse_threshold=thr)
         self._pairs += idx.shape[0] * idx.shape[1]

@@ -133,6 +133,29 @@ long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
                     float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
                     int64_t *n_refined);
 
+/* Coarse screen precision: 0 = FP32 (the band-point flat filter, exact
+ * reference semantics), 1 = fp16 or 2 = bf16 matrix cores (band 512 only;
+ * FP32 accumulation), whose statistic is widened by (1 + delta) at the gate
+ * so the survivors are a superset of the FP32 screen's when its relative
+ * error stays within delta.  Other bands always use FP32.
+ * Replaces: the FP16 coarse pass of the reference's GPU path (fusedTierB with
+ * COARSE16, src/gpu/tierb.slang:776-900) -- no exact counterpart. */
+int mfh_hier_set_coarse_precision(mfh_ctx *ctx, int precision, float delta);
+
+/* The coarse statistic max |z_c| per pair over the coarse window of [ws, we)
+ * at the given precision, into host_max[nd*nt]; for calibrating delta and
+ * for tests.  No refine runs.
+ * Replaces: the coarse dump of MF_HMF_DUMP (src/hmf.c:67-68, :279-282). */
+int mfh_hier_coarse_max(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t ws, size_t we,
+                        int precision, float *host_max);
+
+/* Run one MFMA instruction on given matrices (row-major FP32 inputs converted
+ * to the instruction's type): variant 0 = 16x16x32 f16, 1 = 16x16x32 bf16
+ * (A 16x32, B 32x16), 2 = 16x16x16 f16, 3 = 16x16x16 bf16 (A, B 16x16);
+ * C is 16x16.  Pins the fragment layouts the coarse screen relies on.
+ * Replaces: nothing in the reference (a self-test of this build). */
+int mfh_selftest_mfma(int device, int variant, const float *A, const float *B, float *C);
+
 /* Same with device outputs, queued on the stream; if host_refined is not
  * NULL it receives the survivor count once the stream reaches that point.
  * Replaces: the one-command-buffer hierarchical recording of

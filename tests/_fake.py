@@ -36,6 +36,9 @@ class FakeContext:
     def hier_set_band(self, band, ref_fraction=-1.0):
         self.band, self.ref_f = band, ref_fraction
 
+    def hier_set_coarse_precision(self, precision, delta):
+        self.coarse_precision = (precision, delta)
+
     def hier_peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, coarse_threshold,
                    counts=False):
         FakeContext.calls.append(("hier", d0, nd, t0, nt, coarse_threshold))

@@ -39,11 +39,14 @@ bool hip_ok(hipError_t e, const char *what) {
 
 Ctx::~Ctx() {
   if (dev >= 0) hipSetDevice(dev);
+  if (stream) hipStreamSynchronize(stream);
+  if (graph_exec) hipGraphExecDestroy(graph_exec);
   for (auto &kv : bufs) hipFree(kv.second.p);
   for (auto &e : prof_events) {
     hipEventDestroy(e.a);
     hipEventDestroy(e.b);
   }
+  for (auto e : event_pool) hipEventDestroy(e);
   if (stream) hipStreamDestroy(stream);
 }
 
@@ -66,21 +69,38 @@ void *Ctx::scratch(const char *name, size_t bytes) {
     hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
+    // a captured graph may hold the old address
+    if (graph_exec) {
+      hipGraphExecDestroy(graph_exec);
+      graph_exec = nullptr;
+      graph_key.clear();
+    }
   }
   if (!hip_ok(hipMalloc(&b.p, bytes ? bytes : 1), "hipMalloc")) return nullptr;
   b.bytes = bytes;
   return b.p;
 }
 
-// Event pair around one launch, when profiling is on.
+hipEvent_t Ctx::take_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+// Event pair around one launch, when profiling is on (events are pooled).
 Ctx::ProfScope::ProfScope(Ctx *c, const char *name) : c_(c), name_(name) {
-  if (!c_->prof_on) return;
-  hipEventCreate(&ev_.a);
-  hipEventCreate(&ev_.b);
+  if (!c_->prof_on || c_->capturing) return;
+  ev_.a = c_->take_event();
+  ev_.b = c_->take_event();
   hipEventRecord(ev_.a, c_->stream);
 }
 Ctx::ProfScope::~ProfScope() {
-  if (!c_->prof_on) return;
+  if (!c_->prof_on || c_->capturing) return;
   hipEventRecord(ev_.b, c_->stream);
   ev_.name = name_;
   c_->prof_events.push_back(ev_);
@@ -95,8 +115,8 @@ void Ctx::prof_collect() {
     auto &acc = prof_totals[e.name];
     acc.first += ms;
     acc.second += 1;
-    hipEventDestroy(e.a);
-    hipEventDestroy(e.b);
+    event_pool.push_back(e.a);
+    event_pool.push_back(e.b);
   }
   prof_events.clear();
 }
@@ -282,6 +302,15 @@ extern "C" int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value) {
   if (!strcmp(key, "dpw")) {
     if (value < 0) return fail("dpw must be >= 0");
     c->tune_dpw = (int)value;
+    return 0;
+  }
+  if (!strcmp(key, "graphs")) {
+    c->use_graphs = value != 0;
+    return 0;
+  }
+  if (!strcmp(key, "refine_grid")) {
+    if (value < 1) return fail("refine_grid must be >= 1");
+    c->refine_grid_cap = (int)value;
     return 0;
   }
   return fail("unknown tuning key '%s'", key);

@@ -310,13 +310,80 @@ extern "C" int mfh_hier_coarse_max(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
   return 0;
 }
 
+namespace mfh {
+
+static long long fbits(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  return (long long)u;
+}
+static long long dbits(double x) {
+  long long u;
+  memcpy(&u, &x, 8);
+  return u;
+}
+
+// hier_run through the context's graph cache: the first call with a set of
+// launch parameters runs eagerly, the second identical call is captured into
+// a HIP graph, and later identical calls replay it with one launch.
+int hier_run_cached(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,
+                    float coarse_thr, int64_t *idx, cf *val, int *count_out) {
+  if (!c->use_graphs || c->prof_on)
+    return hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, idx, val, count_out, nullptr);
+  hipSetDevice(c->dev);
+  if (c->coarse_band > 0 && hier_prepare(c) < 0) return -1;
+  std::vector<long long> key = {d0, nd, t0, nt, (long long)binsize, fbits(threshold), (long long)ws, (long long)we,
+                                fbits(coarse_thr), (long long)(intptr_t)idx, (long long)(intptr_t)val,
+                                c->coarse_band, dbits(c->coarse_ref_f), c->coarse_prec, fbits(c->coarse_delta),
+                                c->refine_grid_cap, (long long)(intptr_t)c->coarse_tmpl};
+  if (c->graph_exec && key == c->graph_key) {
+    if (!hip_ok(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch")) return -1;
+  } else if (key == c->last_key) {
+    if (c->graph_exec) {
+      hipGraphExecDestroy(c->graph_exec);
+      c->graph_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    if (!hip_ok(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+      return -1;
+    c->capturing = true;
+    const int rc = hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, idx, val, nullptr, nullptr);
+    c->capturing = false;
+    const bool ended = hip_ok(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
+    if (rc < 0 || !ended) {
+      if (g) hipGraphDestroy(g);
+      return -1;
+    }
+    hipGraphExec_t ex = nullptr;
+    const bool inst = hip_ok(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    hipGraphDestroy(g);
+    if (!inst) return -1;
+    c->graph_exec = ex;
+    c->graph_key = key;
+    if (!hip_ok(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch")) return -1;
+  } else {
+    if (hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, idx, val, nullptr, nullptr) < 0)
+      return -1;
+    c->last_key = key;
+  }
+  if (count_out) {
+    int *count = (int *)c->scratch("refine_count", sizeof(int));
+    if (!count || !hip_ok(hipMemcpyAsync(count_out, count, sizeof(int), hipMemcpyDeviceToHost, c->stream),
+                          "copy count"))
+      return -1;
+  }
+  return 0;
+}
+
+}  // namespace mfh
+
 extern "C" int mfh_hier_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
                                     size_t ws, size_t we, float coarse_thr, int64_t *dev_idx, void *dev_val_c64,
                                     int32_t *host_refined) {
   if (!ctx) return fail("null context");
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  return mfh::hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, dev_idx, (mfh::cf *)dev_val_c64,
-                       host_refined, nullptr);
+  return mfh::hier_run_cached(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, dev_idx,
+                              (mfh::cf *)dev_val_c64, host_refined);
 }
 
 extern "C" long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
@@ -335,7 +402,7 @@ extern "C" long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, siz
   mfh::cf *dval = (mfh::cf *)c->scratch("out_val", cells * sizeof(mfh::cf));
   if (!didx || !dval) return -1;
   int refined = 0;
-  if (mfh::hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, didx, dval, &refined, nullptr) < 0)
+  if (mfh::hier_run_cached(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, didx, dval, &refined) < 0)
     return -1;
   if (!hip_ok(hipMemcpyAsync(idx, didx, cells * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream), "copy idx") ||
       !hip_ok(hipMemcpyAsync(val_c64, dval, cells * sizeof(mfh::cf), hipMemcpyDeviceToHost, c->stream),

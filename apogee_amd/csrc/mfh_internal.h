@@ -53,7 +53,16 @@ struct Ctx {
     const char *name = "";
   };
   bool prof_on = false;
+  bool capturing = false;  // a stream capture is recording launches
   std::vector<Ev> prof_events;
+  std::vector<hipEvent_t> event_pool;
+  hipEvent_t take_event();
+  // HIP graph of the hierarchical stage sequence, keyed by its launch
+  // parameters: replayed when a call repeats them (captured on the second
+  // identical call, so every buffer and kernel attribute exists by then).
+  bool use_graphs = true;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<long long> graph_key, last_key;
   std::map<std::string, std::pair<double, long>> prof_totals;
   void prof_collect();
   struct ProfScope {

@@ -24,9 +24,10 @@ from . import MatchedFilter, _from_any
 #: Coarse-screen precisions: name -> (code of mfh_hier_set_coarse_precision,
 #: default gate widening).  The widening bounds the relative error of the
 #: matrix-core statistic against the FP32 screen with headroom: the largest
-#: error measured on noise, loud and chirp-bank pairs (tests/test_gpu_mfma.py)
-#: was ~1.5e-3 (fp16) and ~1.2e-2 (bf16); the defaults are 4x and 4x that.
-_COARSE_PRECISIONS = {"fp32": (0, 0.0), "fp16": (1, 6e-3), "bf16": (2, 5e-2)}
+#: error measured on noise, loud and inspiral-bank pairs across windows
+#: (tests/test_gpu_mfma.py) was 4.9e-4 (fp16) and 4.1e-3 (bf16); the
+#: defaults are about 4x that.
+_COARSE_PRECISIONS = {"fp32": (0, 0.0), "fp16": (1, 2e-3), "bf16": (2, 1.6e-2)}
 
 #: Coarse band per transform length on gfx950 (power of two, 64 <= band < n).
 #: At n = 4096 the 512-point band is what the benchmark configuration pins.

@@ -176,7 +176,7 @@ extern "C" mfh_ctx *mfh_create(int device, size_t n, int nd, int nt) {
   }
   c->data = (cf *)c->scratch("data", (size_t)nd * n * sizeof(cf));
   c->tmpl = (cf *)c->scratch("tmpl", (size_t)nt * n * sizeof(cf));
-  if (!c->data || !c->tmpl || !c->twiddles(n)) {
+  if (!c->data || !c->tmpl || (n <= 16384 && !c->twiddles(n))) {
     delete c;
     return nullptr;
   }

@@ -1,0 +1,280 @@
+// Flat filter for n = 32768 and 65536: a four-step split across two kernels.
+//
+// A length-n transform no longer fits one workgroup's LDS (256/512 KiB), so
+// n = N1 x 256 (N1 = 128 or 256) with f = f1 + N1 f2 and k = k2 + 256 k1:
+//   Y[f1][k2] = w_n^(f1 k2) sum_f2 P[f1 + N1 f2] w_256^(f2 k2)     kernel A
+//   z[k2 + 256 k1] = sum_f1 Y[f1][k2] w_N1^(f1 k1)                 kernel B
+// Kernel A forms the product D*conj(T) on load, runs 256-point inverse FFTs
+// over 16 columns per workgroup and writes Y k2-major to scratch (each
+// 16-lane group writes 128 contiguous bytes).  Kernel B gives each pair one
+// workgroup that walks all k2 in groups, runs N1-point inverse FFTs, and keeps
+// a per-bin (key, value) table in LDS: keys rise by ds_max_u64, and after
+// each group the lane holding the bin's key writes its value, so the table
+// always pairs the largest |z|^2 (lowest lag among equals) with its own z.
+// Only the final per-bin result leaves the chip.
+//
+// Reference behaviour: the same selection rule as the flat kernel
+// (src/balanced-inl.h:876-969 in the reference checkout); the reference's GPU
+// carried these lengths in one workgroup with R = 32/64 points per thread
+// (src/gpu/tierb.slang:13-22), which wave64 register budgets do not allow.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "mfh_flat.h"
+#include "mfh_internal.h"
+
+namespace mfh {
+
+namespace {
+
+struct BigArgs {
+  const cf *data, *tmpl;
+  long long data_stride, tmpl_stride;
+  int nd, nt;           // tiled mode: pair p = p0 + blockIdx -> (p / nt, p % nt)
+  long long p0;         // first pair (tiled) or list position (list) of this chunk
+  const int2 *list;     // list mode
+  const int *list_count;
+  cf *scratch;          // [chunk pair][256][N1]
+  const cf *tw256, *twN1;  // Stockham tables of the two lengths
+  const cf *rootH, *rootL; // w_N1^h (h < N1) and w_n^l (l < 256)
+  int64_t *out_idx;
+  cf *out_val;
+  uint32_t ws, we, binsize, nb, ostride;
+  int bshift;
+  float inv_bs, seed;
+};
+
+__device__ __forceinline__ bool big_pair(const BigArgs &a, long long p, int &d, int &t) {
+  if (a.list) {
+    if (p >= *a.list_count) return false;
+    const int2 pr = a.list[p];
+    d = pr.x;
+    t = pr.y;
+    return true;
+  }
+  if (p >= (long long)a.nd * a.nt) return false;
+  d = (int)(p / a.nt);
+  t = (int)(p % a.nt);
+  return true;
+}
+
+// Columns: 16 columns f1 of one pair per workgroup; lane = 16 * t + s, so the
+// 16 lanes of a row group touch 16 consecutive columns (128-byte accesses).
+template <int N1>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) big_cols_kernel(BigArgs a) {
+  constexpr int N = N1 * 256;
+  using S256 = FlatShape<256>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int s = threadIdx.x & 15;
+  const int t = threadIdx.x >> 4;
+  cf *xch = reinterpret_cast<cf *>(smem) + s * S256::XCH;
+  const long long pl = blockIdx.y;  // pair within the chunk
+  int d = 0, tt = 0;
+  const bool valid = big_pair(a, a.p0 + pl, d, tt);  // uniform across the workgroup
+  if (!valid) return;                                 // no barrier has run yet
+  const int f1 = blockIdx.x * 16 + s;
+  const cf *D = a.data + (long long)d * a.data_stride;
+  const cf *T = a.tmpl + (long long)tt * a.tmpl_stride;
+  cf v[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int f = f1 + N1 * (t + 16 * u);
+    v[u] = cmul_conj(D[f], T[f]);
+  }
+  inverse_fft<256>(v, t, xch, a.tw256);
+  cf *Y = a.scratch + pl * N;
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int k2 = t + 16 * u;
+    const int m = f1 * k2;                         // < N
+    const cf w = cmul(a.rootH[m >> 8], a.rootL[m & 255]);
+    Y[(long long)k2 * N1 + f1] = cmul(v[u], w);
+  }
+}
+
+// Rows: one workgroup per pair, SLOTS rows (k2) at a time.
+template <int N1>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) big_rows_kernel(BigArgs a) {
+  using S = FlatShape<N1>;
+  constexpr int TPT = S::TPT;
+  constexpr int SLOTS = S::SLOTS;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cf *xch_all = reinterpret_cast<cf *>(smem);
+  uint64_t *bkey = reinterpret_cast<uint64_t *>(xch_all + SLOTS * S::XCH);
+  cf *bval = reinterpret_cast<cf *>(bkey + a.nb);
+  const int slot = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  cf *xch = xch_all + slot * S::XCH;
+  const long long pl = blockIdx.x;
+  int d = 0, tt = 0;
+  if (!big_pair(a, a.p0 + pl, d, tt)) return;  // uniform; before any barrier
+  const int nb = (int)a.nb;
+  for (int b = threadIdx.x; b < nb; b += 256) bkey[b] = 0ull;
+  __syncthreads();
+  const cf *Y = a.scratch + pl * (long long)(N1 * 256);
+  FlatArgs fa;  // for bin_of / make_key
+  fa.ws = a.ws;
+  fa.we = a.we;
+  fa.binsize = a.binsize;
+  fa.bshift = a.bshift;
+  fa.inv_bs = a.inv_bs;
+  for (int g = 0; g < 256 / SLOTS; g++) {
+    const int k2 = g * SLOTS + slot;
+    cf v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = Y[(long long)k2 * N1 + t + TPT * u];
+    inverse_fft<N1>(v, t, xch, a.twN1);
+    // thread t holds z[k2 + 256 (t + TPT u)]; lags rise with u
+    uint64_t acc = 0ull;
+    int acc_bin = -1;
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const uint32_t k = (uint32_t)(k2 + 256 * (t + TPT * u));
+      if (k >= a.ws && k < a.we) {
+        const uint64_t key = make_key(v[u], k, a.seed);
+        const int b = (int)bin_of(k, fa);
+        if (b != acc_bin) {
+          if (acc) atomicMax((unsigned long long *)&bkey[acc_bin], (unsigned long long)acc);
+          acc = 0ull;
+          acc_bin = b;
+        }
+        acc = key > acc ? key : acc;
+      }
+    }
+    if (acc) atomicMax((unsigned long long *)&bkey[acc_bin], (unsigned long long)acc);
+    __syncthreads();
+    // the holder of each bin's key records its value
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const uint32_t k = (uint32_t)(k2 + 256 * (t + TPT * u));
+      if (k >= a.ws && k < a.we) {
+        const uint64_t key = make_key(v[u], k, a.seed);
+        if (key) {
+          const uint32_t b = bin_of(k, fa);
+          if (bkey[b] == key) bval[b] = v[u];
+        }
+      }
+    }
+    __syncthreads();  // values settled before the next group raises keys; xch reusable
+  }
+  const long long row = (long long)d * a.nt + tt;
+  int64_t *oidx = a.out_idx + row * a.ostride;
+  cf *oval = a.out_val + row * a.ostride;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    const uint64_t key = bkey[b];
+    if (key) {
+      oidx[b] = (int64_t)(~(uint32_t)(key & 0xffffffffull));
+      oval[b] = bval[b];
+    } else {
+      oidx[b] = -1;
+      oval[b] = cf{0.f, 0.f};
+    }
+  }
+}
+
+__global__ void roots_kernel(cf *out, int L, int period) {
+  // out[k] = exp(+2 pi i k / period), k < L
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < L) {
+    double sn, cs;
+    sincospi(2.0 * (double)k / (double)period, &sn, &cs);
+    out[k] = cf{(float)cs, (float)sn};
+  }
+}
+
+constexpr size_t kLds = 160 * 1024;
+
+template <int N1>
+size_t rows_max_bins() {
+  using S = FlatShape<N1>;
+  return (kLds - (size_t)S::SLOTS * S::XCH * sizeof(cf)) / (sizeof(uint64_t) + sizeof(cf));
+}
+
+template <int N1>
+int launch_big_n1(Ctx *c, const FlatRun &r) {
+  constexpr int N = N1 * 256;
+  using S = FlatShape<N1>;
+  BigArgs a;
+  a.data = r.data;
+  a.tmpl = r.tmpl;
+  a.data_stride = r.data_stride;
+  a.tmpl_stride = r.tmpl_stride;
+  a.nd = r.nd;
+  a.nt = r.nt;
+  a.list = r.list;
+  a.list_count = r.list_count;
+  a.tw256 = c->twiddles(256);
+  a.twN1 = c->twiddles(N1);
+  char rname[32];
+  snprintf(rname, sizeof rname, "bigroots%d", N);
+  cf *roots = (cf *)c->scratch(rname, (N1 + 256) * sizeof(cf));
+  if (!a.tw256 || !a.twN1 || !roots) return -1;
+  if (!c->big_roots_ready[N1 == 256]) {
+    hipLaunchKernelGGL(roots_kernel, dim3((N1 + 255) / 256), dim3(256), 0, c->stream, roots, N1, N1);
+    hipLaunchKernelGGL(roots_kernel, dim3(1), dim3(256), 0, c->stream, roots + N1, 256, N);
+    if (!hip_ok(hipGetLastError(), "roots_kernel")) return -1;
+    c->big_roots_ready[N1 == 256] = true;
+  }
+  a.rootH = roots;
+  a.rootL = roots + N1;
+  const long long total = r.list ? (long long)r.list_cap : (long long)r.nd * r.nt;
+  // scratch in chunks of pairs: at most c->big_chunk_bytes of intermediate data
+  const long long chunk = std::max<long long>(1, std::min<long long>(total, c->big_chunk_bytes / ((long long)N * 8)));
+  cf *scr = (cf *)c->scratch("big_scratch", (size_t)chunk * N * sizeof(cf));
+  if (!scr) return -1;
+  a.scratch = scr;
+  const size_t lds_cols = 16 * FlatShape<256>::XCH * sizeof(cf);
+  const float thr = r.threshold;
+  a.seed = thr > 0.f ? (float)((double)thr * (double)thr) : 0.f;
+  a.binsize = (uint32_t)r.binsize;
+  const bool pow2 = (r.binsize & (r.binsize - 1)) == 0;
+  a.bshift = pow2 ? ilog2((int)r.binsize) : -1;
+  a.inv_bs = 1.0f / (float)r.binsize;
+  const size_t nb_total = (r.we - r.ws + r.binsize - 1) / r.binsize;
+  const size_t maxb = rows_max_bins<N1>();
+  static bool attr_set = false;
+  if (!attr_set) {
+    if (!hip_ok(hipFuncSetAttribute((const void *)big_cols_kernel<N1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds_cols),
+                "hipFuncSetAttribute") ||
+        !hip_ok(hipFuncSetAttribute((const void *)big_rows_kernel<N1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kLds),
+                "hipFuncSetAttribute"))
+      return -1;
+    attr_set = true;
+  }
+  for (long long p0 = 0; p0 < total; p0 += chunk) {
+    const long long np = std::min(chunk, total - p0);
+    a.p0 = p0;
+    {
+      Ctx::ProfScope ps(c, r.list ? "refine" : r.prof_name);
+      hipLaunchKernelGGL(big_cols_kernel<N1>, dim3(N1 / 16, (unsigned)np), dim3(256), lds_cols, c->stream, a);
+      if (!hip_ok(hipGetLastError(), "big_cols_kernel")) return -1;
+      for (size_t b0 = 0; b0 < nb_total; b0 += maxb) {
+        const size_t cnt = std::min(maxb, nb_total - b0);
+        a.ws = (uint32_t)(r.ws + b0 * r.binsize);
+        a.we = (uint32_t)std::min(r.we, r.ws + (b0 + cnt) * r.binsize);
+        a.nb = (uint32_t)cnt;
+        a.ostride = (uint32_t)nb_total;
+        a.out_idx = r.out_idx + b0;
+        a.out_val = r.out_val + b0;
+        const size_t lds = (size_t)S::SLOTS * S::XCH * sizeof(cf) + cnt * (sizeof(uint64_t) + sizeof(cf));
+        hipLaunchKernelGGL(big_rows_kernel<N1>, dim3((unsigned)np), dim3(256), lds, c->stream, a);
+        if (!hip_ok(hipGetLastError(), "big_rows_kernel")) return -1;
+      }
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+int launch_big(Ctx *c, size_t n, const FlatRun &r) {
+  if (n == 32768) return launch_big_n1<128>(c, r);
+  if (n == 65536) return launch_big_n1<256>(c, r);
+  return fail("transform length %zu is not a four-step length", n);
+}
+
+}  // namespace mfh

@@ -33,6 +33,9 @@ struct Ctx {
   int coarse_prec = 0;
   float coarse_delta = 0.f;
   bool coarse_roots_ready = false;
+  // four-step lengths (32768, 65536): roots built, scratch budget per chunk
+  bool big_roots_ready[2] = {false, false};
+  long long big_chunk_bytes = 256ll << 20;
   int refine_grid_cap = 2048;  // workgroups of a list-driven (refine) launch
   // Per-pass twiddle tables of the flat kernel for length n (FP64-generated
   // on first use, kept for the context's lifetime).
@@ -101,6 +104,8 @@ int flat_fill_twiddles(Ctx *c, size_t n, cf *tw);
 // Validates, splits windows whose bin table would not fit LDS, and launches
 // the length-n flat kernel on c->stream.  Returns 0 or -1 (message set).
 int launch_flat(Ctx *c, size_t n, const FlatRun &r);
+// The two-kernel four-step flat filter for n = 32768 and 65536.
+int launch_big(Ctx *c, size_t n, const FlatRun &r);
 // Forward spectra of run_series blocks into out [nblocks][n] (device).
 int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out);
 // MFMA coarse statistic (max |z_c|^2 over [cs, ce)) per pair, band 512.

@@ -76,7 +76,7 @@ def test_config1_cpu_parity_workload(gpu_device, mfcpu):
           % (ex, pk.size, (ci == pk["index"]).mean()))
 
 
-@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384])
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384, 32768, 65536])
 def test_every_length_against_the_oracle(n, gpu_device):
     rng = np.random.default_rng(n)
     D, T = cnoise(rng, (3, n)), cnoise(rng, (5, n))
@@ -345,3 +345,57 @@ def test_many_bins_split_across_launches(gpu_device):
     D, T = cnoise(rng, (1, n)), cnoise(rng, (2, n))
     pk = run(D, T, 3, 0.0)
     assert_peaks_match(D, T, pk["index"], pk["value"], 3, 0.0, context="16384 bs=3")
+
+
+# ------------------------------------------------- four-step lengths
+
+def test_config2b_workload(gpu_device):
+    """n = 65536, 4 data x 128 templates, binsize 1024 (64 bins), threshold 5.5
+    on sigma-normalised |z|; 5% of pairs carry their template at SNR 8."""
+    n, nd, nt = 65536, 4, 128
+    rng = np.random.default_rng(20261202 + 22)
+    t_time = rng.standard_normal((nt, n))
+    T = np.fft.fft(t_time)
+    T = (T / np.sqrt(n * np.sum(np.abs(T) ** 2, axis=1, keepdims=True))).astype(np.complex64)
+    x = rng.standard_normal((nd, n))
+    inj = rng.random((nd, nt)) < 0.05
+    lag = {}
+    for d, t in zip(*np.nonzero(inj)):
+        s = int(rng.integers(0, n))
+        lag[(d, t)] = s
+        x[d] += np.roll(t_time[t], s) * 8.0 / np.linalg.norm(t_time[t])
+    D = np.fft.fft(x).astype(np.complex64)
+    pk = run(D, T, 1024, 5.5)
+    assert pk.shape == (nd, nt, 64)
+    assert_peaks_match(D, T, pk["index"], pk["value"], 1024, 5.5, context="config 2b", pair_chunk=64)
+    for (d, t), s in lag.items():
+        assert pk["index"][d, t, s // 1024] == s
+
+
+@pytest.mark.parametrize("n", [32768, 65536])
+@pytest.mark.parametrize("binsize,window", [(None, None), (1000, (37, 30000)), (7, (100, 20000)),
+                                            (4096, (0, 32768))])
+def test_four_step_bins_and_windows(n, binsize, window, gpu_device):
+    rng = np.random.default_rng(n + (binsize or 0))
+    D, T = cnoise(rng, (2, n)), cnoise(rng, (3, n))
+    pk = run(D, T, binsize, 0.0, window)
+    bs = n if binsize is None else binsize
+    assert_peaks_match(D, T, pk["index"], pk["value"], bs, 0.0, window,
+                       context="n=%d bs=%s w=%s" % (n, binsize, window))
+
+
+def test_four_step_known_lags_and_subranges(gpu_device):
+    n = 65536
+    rng = np.random.default_rng(9)
+    x = cnoise(rng, (3, n))
+    lags = [0, 255, 256, 40000, n - 1]
+    T = np.stack([(x[0] * np.exp(2j * np.pi * np.arange(n) * g / n)).astype(np.complex64)
+                  for g in lags])
+    f = mf.MatchedFilter(n, 3, len(lags), device="gpu")
+    f.set_data(x)
+    f.set_templates(T)
+    full = f.run(binsize=n).copy()
+    assert [int(full["index"][0, j, 0]) for j in range(len(lags))] == lags
+    sub = f.run(binsize=n, data=(0, 2), templates=(2, 3))
+    np.testing.assert_array_equal(sub["index"], full["index"][0:2, 2:5])
+    np.testing.assert_array_equal(sub["value"], full["value"][0:2, 2:5])

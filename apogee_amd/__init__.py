@@ -32,7 +32,7 @@ PEAK_DTYPE = np.dtype([("index", "<i8"), ("value", "<c8")])
 _GPU_SIZES = frozenset((64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384,
                         32768, 65536))
 
-__all__ = ["MatchedFilter", "HierarchicalFilter", "PEAK_DTYPE", "devices",
+__all__ = ["MatchedFilter", "HierarchicalFilter", "RefineQueue", "PEAK_DTYPE", "devices",
            "Device", "UnsupportedSize", "__version__"]
 
 #: DLPack device types read as host memory: CPU, CUDA host, ROCm host.
@@ -345,6 +345,7 @@ class MatchedFilter:
 
 
 from .hierarchical import HierarchicalFilter  # noqa: E402
+from .refine import RefineQueue  # noqa: E402
 
 
 def __getattr__(name):

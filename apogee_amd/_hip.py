@@ -73,6 +73,22 @@ def _declare(lib):
     lib.mfh_prof_enable.argtypes = [_vp, ctypes.c_int]
     lib.mfh_prof_read.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_long)]
+    lib.mfh_queue_create.restype = _vp
+    lib.mfh_queue_create.argtypes = [ctypes.c_int]
+    lib.mfh_queue_destroy.restype = None
+    lib.mfh_queue_destroy.argtypes = [_vp]
+    lib.mfh_queue_set_pool.argtypes = [_vp, _vp, _c_size, ctypes.c_int]
+    lib.mfh_queue_plan.restype = ctypes.c_longlong
+    lib.mfh_queue_plan.argtypes = [_vp, ctypes.c_int] + [_vp] * 9
+    lib.mfh_queue_run_async.argtypes = [_vp, _vp, _vp]
+    lib.mfh_queue_run.restype = ctypes.c_longlong
+    lib.mfh_queue_run.argtypes = [_vp, _vp, _vp, _vp]
+    lib.mfh_queue_set_tuning.argtypes = [_vp, ctypes.c_char_p, ctypes.c_long]
+    lib.mfh_queue_sync.argtypes = [_vp]
+    lib.mfh_queue_stream.restype = _vp
+    lib.mfh_queue_stream.argtypes = [_vp]
+    lib.mfh_queue_prof_enable.argtypes = [_vp, ctypes.c_int]
+    lib.mfh_queue_prof_read.argtypes = [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
     return lib
 
 
@@ -328,4 +344,95 @@ class Context:
         cnt = ctypes.c_long(0)
         check(lib().mfh_prof_read(self.handle, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)),
               "mfh_prof_read")
+        return ms.value, cnt.value
+
+
+class Queue:
+    """Device side of :class:`apogee_amd.refine.RefineQueue`: a spectrum pool,
+    a planned work queue and the persistent refine launch that drains it."""
+
+    def __init__(self, index):
+        L = lib()
+        self.index = int(index)
+        h = L.mfh_queue_create(self.index)
+        if not h:
+            raise RuntimeError("mfh_queue_create failed: %s" % last_error())
+        self._h = ctypes.c_void_p(h)
+        self._pool_ref = None
+
+    def destroy(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.mfh_queue_destroy(h)
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise RuntimeError("queue destroyed")
+        return self._h
+
+    def set_pool(self, pool):
+        """pool: 1-D complex64 host array (copied) or ROCm tensor (referenced)."""
+        dev = _device_pointer(pool)
+        if dev is not None:
+            import torch
+            torch.cuda.current_stream(pool.device).synchronize()
+            check(lib().mfh_queue_set_pool(self.handle, ctypes.c_void_p(dev), int(pool.numel()), 1),
+                  "mfh_queue_set_pool")
+            self._pool_ref = pool     # the queue reads it in place
+        else:
+            check(lib().mfh_queue_set_pool(self.handle, _ptr(pool), int(pool.size), 0),
+                  "mfh_queue_set_pool")
+            self._pool_ref = None
+
+    def plan(self, n, data_off, tmpl_off, ncand, binsize, threshold, ws, we):
+        """Returns (per-request output offsets int64, total output elements)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.int64) for a in (n, data_off, tmpl_off)]
+        nc = np.ascontiguousarray(ncand, dtype=np.int32)
+        bs = np.ascontiguousarray(binsize, dtype=np.int64)
+        th = np.ascontiguousarray(threshold, dtype=np.float32)
+        w0 = np.ascontiguousarray(ws, dtype=np.int64)
+        w1 = np.ascontiguousarray(we, dtype=np.int64)
+        nreq = int(nc.size)
+        off = np.empty(nreq, dtype=np.int64)
+        total = lib().mfh_queue_plan(self.handle, nreq, _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]),
+                                     _ptr(nc), _ptr(bs), _ptr(th), _ptr(w0), _ptr(w1), _ptr(off))
+        check(total, "mfh_queue_plan")
+        return off, int(total)
+
+    def run(self, nout, npairs):
+        """Host results: (index int64 [nout], value complex64 [nout], counts int32 [npairs])."""
+        idx = np.empty(max(nout, 1), dtype=np.int64)
+        val = np.empty(max(nout, 1), dtype=np.complex64)
+        cnt = np.empty(max(npairs, 1), dtype=np.int32)
+        check(lib().mfh_queue_run(self.handle, _ptr(idx), _ptr(val), _ptr(cnt)), "mfh_queue_run")
+        return idx[:nout], val[:nout], cnt[:npairs]
+
+    def run_async(self, idx_ptr, val_ptr):
+        check(lib().mfh_queue_run_async(self.handle, ctypes.c_void_p(int(idx_ptr)),
+                                        ctypes.c_void_p(int(val_ptr))), "mfh_queue_run_async")
+
+    def sync(self):
+        check(lib().mfh_queue_sync(self.handle), "mfh_queue_sync")
+
+    def stream(self):
+        return int(lib().mfh_queue_stream(self.handle) or 0)
+
+    def set_tuning(self, key, value):
+        check(lib().mfh_queue_set_tuning(self.handle, key.encode(), int(value)), "mfh_queue_set_tuning")
+
+    def prof_enable(self, on=True):
+        check(lib().mfh_queue_prof_enable(self.handle, 1 if on else 0), "mfh_queue_prof_enable")
+
+    def prof_read(self):
+        ms = ctypes.c_double(0.0)
+        cnt = ctypes.c_long(0)
+        check(lib().mfh_queue_prof_read(self.handle, ctypes.byref(ms), ctypes.byref(cnt)),
+              "mfh_queue_prof_read")
         return ms.value, cnt.value

@@ -36,6 +36,7 @@ struct BigArgs {
   long long p0;         // first pair (tiled) or list position (list) of this chunk
   const int2 *list;     // list mode
   const int *list_count;
+  const long long *list_ooff;  // list mode: output element offset per pair
   cf *scratch;          // [chunk pair][256][N1]
   const cf *tw256, *twN1;  // Stockham tables of the two lengths
   const cf *rootH, *rootL; // w_N1^h (h < N1) and w_n^l (l < 256)
@@ -159,9 +160,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
     }
     __syncthreads();  // values settled before the next group raises keys; xch reusable
   }
-  const long long row = (long long)d * a.nt + tt;
-  int64_t *oidx = a.out_idx + row * a.ostride;
-  cf *oval = a.out_val + row * a.ostride;
+  const long long o = a.list_ooff ? a.list_ooff[a.p0 + pl] : ((long long)d * a.nt + tt) * a.ostride;
+  int64_t *oidx = a.out_idx + o;
+  cf *oval = a.out_val + o;
   for (int b = threadIdx.x; b < nb; b += 256) {
     const uint64_t key = bkey[b];
     if (key) {
@@ -205,18 +206,11 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
   a.nt = r.nt;
   a.list = r.list;
   a.list_count = r.list_count;
+  a.list_ooff = r.list_ooff;
   a.tw256 = c->twiddles(256);
   a.twN1 = c->twiddles(N1);
-  char rname[32];
-  snprintf(rname, sizeof rname, "bigroots%d", N);
-  cf *roots = (cf *)c->scratch(rname, (N1 + 256) * sizeof(cf));
+  const cf *roots = big_roots(c, N1);
   if (!a.tw256 || !a.twN1 || !roots) return -1;
-  if (!c->big_roots_ready[N1 == 256]) {
-    hipLaunchKernelGGL(roots_kernel, dim3((N1 + 255) / 256), dim3(256), 0, c->stream, roots, N1, N1);
-    hipLaunchKernelGGL(roots_kernel, dim3(1), dim3(256), 0, c->stream, roots + N1, 256, N);
-    if (!hip_ok(hipGetLastError(), "roots_kernel")) return -1;
-    c->big_roots_ready[N1 == 256] = true;
-  }
   a.rootH = roots;
   a.rootL = roots + N1;
   const long long total = r.list ? (long long)r.list_cap : (long long)r.nd * r.nt;
@@ -270,6 +264,21 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
 }
 
 }  // namespace
+
+const cf *big_roots(Ctx *c, int n1) {
+  const int n = n1 * 256;
+  char rname[32];
+  snprintf(rname, sizeof rname, "bigroots%d", n);
+  cf *roots = (cf *)c->scratch(rname, (n1 + 256) * sizeof(cf));
+  if (!roots) return nullptr;
+  if (!c->big_roots_ready[n1 == 256]) {
+    hipLaunchKernelGGL(roots_kernel, dim3((n1 + 255) / 256), dim3(256), 0, c->stream, roots, n1, n1);
+    hipLaunchKernelGGL(roots_kernel, dim3(1), dim3(256), 0, c->stream, roots + n1, 256, n);
+    if (!hip_ok(hipGetLastError(), "roots_kernel")) return nullptr;
+    c->big_roots_ready[n1 == 256] = true;
+  }
+  return roots;
+}
 
 int launch_big(Ctx *c, size_t n, const FlatRun &r) {
   if (n == 32768) return launch_big_n1<128>(c, r);

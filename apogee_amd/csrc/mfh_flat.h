@@ -40,6 +40,7 @@ struct FlatArgs {
   int dpw;             // data rows per workgroup (tiled mode)
   const int2 *list;    // list mode: (data row, template row) per pair
   const int *list_count;  // list mode: number of pairs in the list
+  const long long *list_ooff;  // list mode: output element offset per pair (null: row-major d*nt + t)
   uint32_t ws, we;     // lag window [ws, we)
   uint32_t binsize, nb;  // nb: bins of this launch
   uint32_t ostride;      // output elements between pairs (>= nb)
@@ -324,15 +325,15 @@ flat_kernel(FlatArgs a) {
       }
       __syncthreads();
       if (valid) {
-        const long long row = (long long)d * a.nt + tt;
+        const long long o = (LIST && a.list_ooff) ? a.list_ooff[dd + slot] : ((long long)d * a.nt + tt) * a.ostride;
         const uint64_t win = bins[0];
         if (key && key == win) {
-          a.out_idx[row * a.ostride] = (int64_t)kb;
-          a.out_val[row * a.ostride] = bz;
+          a.out_idx[o] = (int64_t)kb;
+          a.out_val[o] = bz;
         }
         if (t == 0 && win == 0ull) {
-          a.out_idx[row * a.ostride] = -1;
-          a.out_val[row * a.ostride] = cf{0.f, 0.f};
+          a.out_idx[o] = -1;
+          a.out_val[o] = cf{0.f, 0.f};
         }
       }
     } else {
@@ -396,9 +397,9 @@ flat_kernel(FlatArgs a) {
 
       // ---- the owner of each winning lag writes it; empty bins get -1/0 ----
       if (valid) {
-        const long long row = (long long)d * a.nt + tt;
-        int64_t *oidx = a.out_idx + row * a.ostride;
-        cf *oval = a.out_val + row * a.ostride;
+        const long long o = (LIST && a.list_ooff) ? a.list_ooff[dd + slot] : ((long long)d * a.nt + tt) * a.ostride;
+        int64_t *oidx = a.out_idx + o;
+        cf *oval = a.out_val + o;
 #pragma unroll
         for (int u = 0; u < 16; u++) {
           const uint32_t k = (uint32_t)(t + u * TPT);

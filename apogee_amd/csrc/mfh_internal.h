@@ -93,6 +93,7 @@ struct FlatRun {
   // list mode: pairs (data row, template row) read from the device
   const int2 *list = nullptr;
   const int *list_count = nullptr;
+  const long long *list_ooff = nullptr;  // output element offset per pair (else d*nt + t rows)
   int list_cap = 0;
   const char *prof_name = "flat";
 };
@@ -106,6 +107,9 @@ int flat_fill_twiddles(Ctx *c, size_t n, cf *tw);
 int launch_flat(Ctx *c, size_t n, const FlatRun &r);
 // The two-kernel four-step flat filter for n = 32768 and 65536.
 int launch_big(Ctx *c, size_t n, const FlatRun &r);
+// Four-step root tables for n = n1 * 256: w_n1^h (h < n1) then w_n^l (l < 256),
+// FP64-generated once per context.
+const cf *big_roots(Ctx *c, int n1);
 // Forward spectra of run_series blocks into out [nblocks][n] (device).
 int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out);
 // MFMA coarse statistic (max |z_c|^2 over [cs, ce)) per pair, band 512.

@@ -207,6 +207,44 @@ int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value);
 int mfh_prof_enable(mfh_ctx *ctx, int on);
 int mfh_prof_read(mfh_ctx *ctx, const char *name, double *total_ms, long *count);
 
+/* ---- Uneven batched refine over a device work queue ---------------------
+ * Requests of mixed transform lengths (64 .. 65536), each one data spectrum
+ * and c consecutive template spectra in a shared pool of complex64, are
+ * drained by one persistent launch.  Each (request, candidate) pair gives the
+ * same index and value as mfh_flat_peaks on that pair.
+ * Replaces: a loop of ap_mf_run / Context.peaks calls, one per request
+ * (python/matchedfilter/matchedfilter.h:66; python/matchedfilter/_vkcompute.py:962),
+ * and the single-length refine list of Context.hier_peaks
+ * (python/matchedfilter/_vkcompute.py:534-850). */
+typedef struct mfh_queue mfh_queue;
+mfh_queue *mfh_queue_create(int device);
+void mfh_queue_destroy(mfh_queue *queue);
+/* The pool: `elems` complex64 values.  on_device = 0 copies host memory into
+ * the queue; 1 uses the caller's device pointer, which must stay valid.
+ * Setting a pool drops the plan (plan again before running). */
+int mfh_queue_set_pool(mfh_queue *queue, const void *spectra_c64, size_t elems, int on_device);
+/* Plan nreq requests.  Request r: length n[r]; data spectrum at pool element
+ * data_off[r]; ncand[r] templates at tmpl_off[r] + j*n[r]; bins of binsize[r]
+ * over the lag window [ws[r], we[r]); threshold[r] as in mfh_flat_peaks.
+ * out_off[r] (may be null) receives the element offset of the request's
+ * [ncand][nbins] outputs.  Returns the total output elements or -1. */
+long long mfh_queue_plan(mfh_queue *queue, int nreq, const int64_t *n, const int64_t *data_off,
+                         const int64_t *tmpl_off, const int32_t *ncand, const int64_t *binsize,
+                         const float *threshold, const int64_t *ws, const int64_t *we, int64_t *out_off);
+/* Run the plan into device outputs, queued on the queue's stream.  The
+ * second identical call is captured into a HIP graph and replayed after. */
+int mfh_queue_run_async(mfh_queue *queue, int64_t *dev_idx, void *dev_val_c64);
+/* Run into host outputs; counts (may be null) receives the reported bins per
+ * pair in plan order.  Returns the number of reported bins or -1. */
+long long mfh_queue_run(mfh_queue *queue, int64_t *idx, void *val_c64, int32_t *counts);
+/* Knobs: "grid_per_cu" (persistent workgroups per CU, default 1), "graphs". */
+int mfh_queue_set_tuning(mfh_queue *queue, const char *key, long value);
+int mfh_queue_sync(mfh_queue *queue);
+void *mfh_queue_stream(mfh_queue *queue);
+/* Event timing of the persistent launch (graph replay disabled meanwhile). */
+int mfh_queue_prof_enable(mfh_queue *queue, int on);
+int mfh_queue_prof_read(mfh_queue *queue, double *total_ms, long *count);
+
 #ifdef __cplusplus
 }
 #endif

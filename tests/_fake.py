@@ -65,3 +65,42 @@ class FakeContext:
         idx, val, _ = O.series_peaks(np.asarray(series), starts, win_start, win_end, T, binsize,
                                      threshold)
         return idx, val.astype(np.complex64), 0
+
+
+class FakeQueue:
+    """Host double of apogee_amd._hip.Queue: plans in Python and answers each
+    pair with the FP64 oracle's flat filter."""
+
+    def __init__(self, index):
+        self.index = index
+        self.pool = None
+        self.plan_args = None
+
+    def set_pool(self, pool):
+        self.pool = np.array(pool, dtype=np.complex64)
+
+    def plan(self, n, data_off, tmpl_off, ncand, binsize, threshold, ws, we):
+        n = np.asarray(n, np.int64)
+        nb = (np.asarray(we) - np.asarray(ws) + np.asarray(binsize) - 1) // np.asarray(binsize)
+        sizes = np.asarray(ncand, np.int64) * nb
+        off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+        self.plan_args = (n, data_off, tmpl_off, ncand, binsize, threshold, ws, we, nb, off)
+        return off, int(sizes.sum())
+
+    def run(self, nout, npairs):
+        n, doff, toff, cand, bs, thr, ws, we, nb, off = self.plan_args
+        idx = np.empty(nout, np.int64)
+        val = np.empty(nout, np.complex64)
+        cnt = []
+        for r in range(n.size):
+            L, c = int(n[r]), int(cand[r])
+            if c == 0:
+                continue
+            D = self.pool[doff[r]:doff[r] + L][None]
+            T = self.pool[toff[r]:toff[r] + c * L].reshape(c, L)
+            i, v, _ = O.flat_peaks(D, T, int(bs[r]), float(thr[r]), (int(ws[r]), int(we[r])))
+            k = c * int(nb[r])
+            idx[off[r]:off[r] + k] = i.reshape(-1)
+            val[off[r]:off[r] + k] = v.reshape(-1)
+            cnt.extend((i[0] >= 0).sum(axis=1).tolist())
+        return idx, val, np.asarray(cnt, np.int32)

@@ -140,6 +140,12 @@ class MatchedFilter:
         self.device = _parse_device(device)
         self._init_state()
         self._gpu = self._context_factory(self.device.index, self.n, self.ndata, self.ntemplates)
+        self._apply_tuning()
+
+    def _apply_tuning(self):
+        """Launch shapes from the gfx950 autotuner table (defaults if none)."""
+        from . import autotune
+        autotune.apply(self._gpu, "flat", self.n, self.ndata, self.ntemplates)
 
     def _init_state(self):
         self._dataset = False

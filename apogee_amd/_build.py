@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmfhip.so")
 
-SOURCES = ["mfh_api.hip", "mfh_flat.hip", "mfh_hier.hip", "mfh_series.hip", "mfh_coarse.hip", "mfh_big.hip", "mfh_queue.hip"]
+SOURCES = ["mfh_api.hip", "mfh_flat.hip", "mfh_hier.hip", "mfh_series.hip", "mfh_coarse.hip", "mfh_big.hip", "mfh_queue.hip", "mfh_grad.hip"]
 ARCH = os.environ.get("MFH_OFFLOAD_ARCH", "gfx950")
 
 

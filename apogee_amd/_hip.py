@@ -73,6 +73,10 @@ def _declare(lib):
     lib.mfh_prof_enable.argtypes = [_vp, ctypes.c_int]
     lib.mfh_prof_read.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                   ctypes.POINTER(ctypes.c_long)]
+    lib.mfh_peak_grad_async.argtypes = [_vp, _vp, ctypes.c_longlong, ctypes.c_int, _vp, ctypes.c_longlong,
+                                         ctypes.c_int, _c_size, _vp, _vp, _vp, _vp]
+    lib.mfh_peak_values_async.argtypes = [_vp, _vp, ctypes.c_longlong, ctypes.c_int, _vp, ctypes.c_longlong,
+                                          ctypes.c_int, _c_size, _vp, _vp]
     lib.mfh_queue_create.restype = _vp
     lib.mfh_queue_create.argtypes = [ctypes.c_int]
     lib.mfh_queue_destroy.restype = None
@@ -331,6 +335,21 @@ class Context:
                                             ctypes.byref(refined)), "mfh_hier_run_series")
         self.last_refinements = int(refined.value)
         return idx, val, int(refined.value)
+
+    # ------------------------------------------------------- derivatives
+    def peak_grad_async(self, data_ptr, nd, tmpl_ptr, nt, nb, idx_ptr, gval_ptr, gdata_ptr, gtmpl_ptr):
+        """Queue dL/dD and/or dL/dT (device pointers; a null output is skipped)."""
+        vp = lambda p: ctypes.c_void_p(int(p)) if p else None
+        check(lib().mfh_peak_grad_async(self.handle, vp(data_ptr), self.n, int(nd), vp(tmpl_ptr), self.n,
+                                        int(nt), int(nb), vp(idx_ptr), vp(gval_ptr), vp(gdata_ptr),
+                                        vp(gtmpl_ptr)), "mfh_peak_grad_async")
+
+    def peak_values_async(self, data_ptr, nd, tmpl_ptr, nt, nb, idx_ptr, out_ptr):
+        """Queue z at stored lags (device pointers)."""
+        vp = lambda p: ctypes.c_void_p(int(p))
+        check(lib().mfh_peak_values_async(self.handle, vp(data_ptr), self.n, int(nd), vp(tmpl_ptr), self.n,
+                                          int(nt), int(nb), vp(idx_ptr), vp(out_ptr)),
+              "mfh_peak_values_async")
 
     # ------------------------------------------------------------- tuning
     def set_tuning(self, key, value):

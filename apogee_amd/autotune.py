@@ -1,0 +1,236 @@
+"""Launch-shape autotuner for the gfx950 kernels, and its lookup table.
+
+Tunables (each a key of the library's tuning interface):
+
+* ``flat`` -- ``dpw``: data rows each workgroup of the tiled flat kernel
+  walks while its template stays in registers (more rows: fewer template
+  reads and fewer, longer workgroups; fewer rows: more workgroups in flight);
+* ``hier`` -- ``refine_grid``: workgroups of the list-driven refine launch,
+  which drains however many pairs the coarse gate passed;
+* ``queue`` -- ``grid_per_cu``: persistent workgroups per CU of the
+  mixed-length refine queue.
+
+``python -m apogee_amd.autotune --out apogee_amd/tuning_gfx950.json`` times
+every candidate on the device with HIP events (median of repeated launches,
+inputs resident) over a fixed set of shapes and writes the winners.  At run
+time :func:`lookup` answers with the entry of the same operation and length
+whose (ndata, ntemplates) is nearest in log scale; with no table or no entry
+the library's built-in defaults stay.
+
+The reference deliberately does no autotuning (tools/hmf_design.py:4); its
+knobs are environment variables read once (src/matchfilt.c:93-107).
+"""
+import json
+import math
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE_PATH = os.path.join(_HERE, "tuning_gfx950.json")
+
+CANDIDATES = {
+    "flat": {"dpw": [1, 2, 4, 8, 16]},
+    "hier": {"refine_grid": [256, 512, 1024, 2048, 4096]},
+    "queue": {"grid_per_cu": [1, 2]},
+}
+
+# shapes the table covers: (n, ndata, ntemplates, binsize)
+FLAT_SHAPES = [
+    (1024, 16, 1024, 1024), (4096, 16, 1024, 4096), (4096, 64, 256, 512), (4096, 4, 4096, 4096),
+    (16384, 16, 256, 16384), (65536, 4, 128, 1024),
+]
+HIER_SHAPES = [(4096, 16, 1024, 4096, 512)]
+QUEUE_WORKLOADS = [("uneven-4000", 4000)]
+
+_table = None
+
+
+def _load():
+    global _table
+    if _table is None:
+        try:
+            with open(TABLE_PATH) as fh:
+                _table = json.load(fh)
+        except (OSError, ValueError):
+            _table = {"entries": []}
+    return _table
+
+
+def lookup(op, n=None, ndata=None, ntemplates=None):
+    """Best known settings {key: value} for an operation and shape (may be {})."""
+    entries = [e for e in _load().get("entries", []) if e.get("op") == op]
+    if n is not None:
+        same = [e for e in entries if e.get("n") == n]
+        entries = same or entries
+    if not entries:
+        return {}
+
+    def dist(e):
+        d = 0.0
+        for key, v in (("ndata", ndata), ("ntemplates", ntemplates), ("n", n)):
+            if v and e.get(key):
+                d += abs(math.log2(v) - math.log2(e[key]))
+        return d
+    return dict(min(entries, key=dist)["best"])
+
+
+def apply(target, op, n=None, ndata=None, ntemplates=None):
+    """Set the looked-up values on anything with ``set_tuning(key, value)``."""
+    best = lookup(op, n, ndata, ntemplates)
+    for k, v in best.items():
+        target.set_tuning(k, int(v))
+    return best
+
+
+# ------------------------------------------------------------- measuring
+
+def _median_ms(fn, reps, ctx, name):
+    import statistics
+    times = []
+    for _ in range(reps):
+        ctx.prof_enable(True)
+        fn()
+        ctx.sync()
+        ms, cnt = ctx.prof_read(name)
+        times.append(ms)
+    ctx.prof_enable(False)
+    return statistics.median(times)
+
+
+def _tune_flat(device, n, nd, nt, binsize, reps, rng):
+    import numpy as np
+    from . import _hip
+    ctx = _hip.Context(device, n, nd, nt)
+    ctx.set_rows("data", 0, (rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))).astype(np.complex64))
+    ctx.set_rows("template", 0, (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64))
+    import torch
+    nb = -(-n // binsize)
+    idx = torch.empty(nd * nt * nb, dtype=torch.int64, device="cuda:%d" % device)
+    val = torch.empty(nd * nt * nb, dtype=torch.complex64, device="cuda:%d" % device)
+    ms = {}
+    for dpw in CANDIDATES["flat"]["dpw"]:
+        if dpw > nd:
+            continue
+        ctx.set_tuning("dpw", dpw)
+        run = lambda: ctx.peaks_async(0, nd, 0, nt, binsize, 0.0, 0, n, idx.data_ptr(), val.data_ptr())
+        run()
+        ctx.sync()
+        ms[dpw] = _median_ms(run, reps, ctx, "flat")
+    best = min(ms, key=ms.get)
+    ctx.destroy()
+    return {"op": "flat", "n": n, "ndata": nd, "ntemplates": nt, "binsize": binsize,
+            "best": {"dpw": best}, "ms": {str(k): round(v, 5) for k, v in ms.items()}}
+
+
+def _tune_hier(device, n, nd, nt, binsize, band, reps, rng):
+    import numpy as np
+    import torch
+    from . import _hip
+    ctx = _hip.Context(device, n, nd, nt)
+    ctx.set_rows("data", 0, (rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))).astype(np.complex64))
+    ctx.set_rows("template", 0, (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64))
+    ctx.hier_set_band(band, -1.0)
+    nb = -(-n // binsize)
+    idx = torch.empty(nd * nt * nb, dtype=torch.int64, device="cuda:%d" % device)
+    val = torch.empty(nd * nt * nb, dtype=torch.complex64, device="cuda:%d" % device)
+    ms = {}
+    # a gate that passes about 1% of noise pairs: the refine launch sees a
+    # short, uneven list, which is what its grid size trades against
+    cmax = np.sort(ctx.hier_coarse_max(0, nd, 0, nt, 0, band, 0).ravel())
+    gate = float(cmax[int(0.99 * cmax.size)])
+    for grid in CANDIDATES["hier"]["refine_grid"]:
+        ctx.set_tuning("refine_grid", grid)
+        run = lambda: ctx.hier_peaks_async(0, nd, 0, nt, binsize, 0.0, 0, n, gate, idx.data_ptr(), val.data_ptr())
+        run()
+        ctx.sync()
+        ctx.prof_enable(True)
+        times = []
+        for _ in range(reps):
+            ctx.prof_enable(True)
+            run()
+            ctx.sync()
+            t = sum(ctx.prof_read(k)[0] for k in ("coarse", "gate", "refine", "zero"))
+            times.append(t)
+        ctx.prof_enable(False)
+        ms[grid] = sorted(times)[len(times) // 2]
+    best = min(ms, key=ms.get)
+    ctx.destroy()
+    return {"op": "hier", "n": n, "ndata": nd, "ntemplates": nt, "binsize": binsize, "band": band,
+            "best": {"refine_grid": best}, "ms": {str(k): round(v, 5) for k, v in ms.items()}}
+
+
+def uneven_workload(nreq, seed, device="cuda"):
+    """The uneven-refine plan: n in {1024, 4096, 16384, 65536} with weights
+    .4/.3/.2/.1, binsize n/8, candidates ~ geometric(1/3); spectra are seeded
+    Gaussian complex64 generated on the device.  Returns (pool, plan kwargs)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    n = rng.choice([1024, 4096, 16384, 65536], size=nreq, p=[0.4, 0.3, 0.2, 0.1]).astype(np.int64)
+    cand = rng.geometric(1.0 / 3.0, size=nreq).astype(np.int64)
+    sizes = n * (1 + cand)
+    doff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    pool = torch.randn(int(sizes.sum()), dtype=torch.complex64, device=device, generator=g)
+    return pool, dict(n=n, data_offset=doff, template_offset=doff + n, candidates=cand, binsize=n // 8)
+
+
+def _tune_queue(device, name, nreq, reps):
+    from .refine import RefineQueue
+    import torch
+    pool, plan = uneven_workload(nreq, 20261204, device="cuda:%d" % device)
+    q = RefineQueue(device="gpu:%d" % device)
+    q.plan(pool, **plan)
+    idx = torch.empty(q.total, dtype=torch.int64, device=pool.device)
+    val = torch.empty(q.total, dtype=torch.complex64, device=pool.device)
+    ms = {}
+    for gpc in CANDIDATES["queue"]["grid_per_cu"]:
+        q.set_tuning("grid_per_cu", gpc)
+        q.run_async(idx, val)
+        q.sync()
+        times = []
+        for _ in range(reps):
+            q._q.prof_enable(True)
+            q.run_async(idx, val)
+            q.sync()
+            times.append(q._q.prof_read()[0])
+        q._q.prof_enable(False)
+        ms[gpc] = sorted(times)[len(times) // 2]
+    best = min(ms, key=ms.get)
+    return {"op": "queue", "workload": name, "requests": nreq, "best": {"grid_per_cu": best},
+            "ms": {str(k): round(v, 5) for k, v in ms.items()}}
+
+
+def main(argv=None):
+    import argparse
+    import numpy as np
+    from . import _hip
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--out", default=TABLE_PATH)
+    ap.add_argument("--reps", type=int, default=7)
+    ap.add_argument("--device", type=int, default=0)
+    args = ap.parse_args(argv)
+    name, arch = _hip.device_info(args.device)
+    rng = np.random.default_rng(7)
+    entries = []
+    for n, nd, nt, bs in FLAT_SHAPES:
+        e = _tune_flat(args.device, n, nd, nt, bs, args.reps, rng)
+        print(json.dumps(e), flush=True)
+        entries.append(e)
+    for n, nd, nt, bs, band in HIER_SHAPES:
+        e = _tune_hier(args.device, n, nd, nt, bs, band, args.reps, rng)
+        print(json.dumps(e), flush=True)
+        entries.append(e)
+    for wname, nreq in QUEUE_WORKLOADS:
+        e = _tune_queue(args.device, wname, nreq, args.reps)
+        print(json.dumps(e), flush=True)
+        entries.append(e)
+    table = {"device": name, "arch": arch, "method": "median of %d event-timed launches" % args.reps,
+             "entries": entries}
+    with open(args.out, "w") as fh:
+        json.dump(table, fh, indent=1)
+    print("wrote", args.out)
+
+
+if __name__ == "__main__":
+    main()

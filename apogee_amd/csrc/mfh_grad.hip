@@ -1,0 +1,176 @@
+// Gradients of reported peak values with respect to the data and template
+// spectra, from the stored peak indices alone (no correlation series).
+//
+// A reported peak of pair (d, t) at lag k has the value
+//     z = sum_f D_d[f] conj(T_t[f]) w^(f k),   w = exp(+2 pi i / n),
+// so, with g = dL/dRe z + i dL/dIm z the incoming gradient of a real loss
+// (PyTorch's convention for complex tensors),
+//     dL/dD_d[f] = sum_{t, bins} g       T_t[f] w^(-f k)
+//     dL/dT_t[f] = sum_{d, bins} conj(g) D_d[f] w^(+f k)
+// Bins that reported nothing (index -1) hold the constant 0 and contribute
+// nothing.  Each output element is one thread's ordered sum (no atomics), so
+// the result is deterministic.  w^m comes from an FP64-generated table of the
+// n roots, indexed by (f k) mod n in exact integer arithmetic.
+//
+// The same table gives the peak value itself at a stored lag (a direct sum
+// over f per peak, one wave each), which is how a caller recomputes z at the
+// argmax without keeping the series.
+//
+// Reference behaviour: the forward value is the flat filter's
+// (src/balanced-inl.h:876-969); the reference has no derivative.
+#include <hip/hip_runtime.h>
+
+#include "mfh_dft.h"
+#include "mfh_internal.h"
+#include "mfhip.h"
+
+namespace mfh {
+
+namespace {
+
+__global__ void unit_circle_kernel(cf *out, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    double sn, cs;
+    sincospi(2.0 * (double)k / (double)n, &sn, &cs);
+    out[k] = cf{(float)cs, (float)sn};
+  }
+}
+
+// dL/dT (TEMPL = true) or dL/dD: one thread per (row, f); the row is
+// blockIdx.y, the other side's rows are summed over.
+template <bool TEMPL>
+__global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ other, long long other_stride,
+                                                        int nother, const int64_t *__restrict__ idx,
+                                                        const cf *__restrict__ gval, int nd, int nt, int nb,
+                                                        const cf *__restrict__ roots, int n, cf *__restrict__ out,
+                                                        long long out_stride) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (f >= n) return;
+  const uint32_t mask = (uint32_t)n - 1u;
+  float ax = 0.f, ay = 0.f;
+  for (int j = 0; j < nother; j++) {
+    // pair (d, t): templates sum over data rows j, data sums over templates j
+    const long long pair = TEMPL ? (long long)j * nt + row : (long long)row * nt + j;
+    const cf x = other[(long long)j * other_stride + f];
+    for (int b = 0; b < nb; b++) {
+      const long long k = idx[pair * nb + b];
+      if (k < 0) continue;
+      const cf g = gval[pair * nb + b];
+      const cf w = roots[((uint32_t)f * (uint32_t)k) & mask];
+      // templates: conj(g) x w ; data: g x conj(w)
+      cf xw;
+      if (TEMPL) {
+        xw = cmul(x, w);
+        ax += g.x * xw.x + g.y * xw.y;
+        ay += g.x * xw.y - g.y * xw.x;
+      } else {
+        xw = cmul_conj(x, w);
+        ax += g.x * xw.x - g.y * xw.y;
+        ay += g.x * xw.y + g.y * xw.x;
+      }
+    }
+  }
+  out[(long long)row * out_stride + f] = cf{ax, ay};
+}
+
+// z at stored lags: one wave per (pair, bin); FP32 products, FP64 sums.
+__global__ void __launch_bounds__(256) peak_value_kernel(const cf *__restrict__ D, long long dstride,
+                                                         const cf *__restrict__ T, long long tstride, int nt,
+                                                         int nb, long long npeaks, const int64_t *__restrict__ idx,
+                                                         const cf *__restrict__ roots, int n, cf *__restrict__ out) {
+  const long long p = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= npeaks) return;  // uniform per wave
+  const long long k = idx[p];
+  const long long pair = p / nb;
+  const int d = (int)(pair / nt), t = (int)(pair % nt);
+  double sx = 0.0, sy = 0.0;
+  if (k >= 0) {
+    const uint32_t mask = (uint32_t)n - 1u;
+    const cf *dp = D + (long long)d * dstride;
+    const cf *tp = T + (long long)t * tstride;
+    for (int f = lane; f < n; f += 64) {
+      const cf pr = cmul(cmul_conj(dp[f], tp[f]), roots[((uint32_t)f * (uint32_t)k) & mask]);
+      sx += pr.x;
+      sy += pr.y;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sx += __shfl_xor(sx, off, 64);
+    sy += __shfl_xor(sy, off, 64);
+  }
+  if (lane == 0) out[p] = k >= 0 ? cf{(float)sx, (float)sy} : cf{0.f, 0.f};
+}
+
+}  // namespace
+
+static const cf *unit_circle(Ctx *c) {
+  char name[32];
+  snprintf(name, sizeof name, "circle%zu", c->n);
+  auto it = c->bufs.find(name);
+  if (it != c->bufs.end() && it->second.p) return (const cf *)it->second.p;
+  cf *r = (cf *)c->scratch(name, c->n * sizeof(cf));
+  if (!r) return nullptr;
+  hipLaunchKernelGGL(unit_circle_kernel, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream, r,
+                     (int)c->n);
+  if (!hip_ok(hipGetLastError(), "unit_circle_kernel")) return nullptr;
+  return r;
+}
+
+}  // namespace mfh
+
+using mfh::cf;
+using mfh::Ctx;
+using mfh::fail;
+using mfh::hip_ok;
+
+extern "C" int mfh_peak_grad_async(mfh_ctx *ctx, const void *dev_data_c64, long long data_stride, int nd,
+                                   const void *dev_tmpl_c64, long long tmpl_stride, int nt, size_t nb,
+                                   const int64_t *dev_idx, const void *dev_gval_c64, void *dev_grad_data_c64,
+                                   void *dev_grad_tmpl_c64) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (nd < 1 || nt < 1 || nb < 1) return fail("empty peak array (%d x %d x %zu)", nd, nt, nb);
+  if (!dev_data_c64 || !dev_tmpl_c64 || !dev_idx || !dev_gval_c64) return fail("null input pointer");
+  if (data_stride < (long long)c->n || tmpl_stride < (long long)c->n) return fail("row stride below n");
+  if (nd > 65535 || nt > 65535) return fail("at most 65535 data and template rows per call");
+  hipSetDevice(c->dev);
+  const cf *roots = mfh::unit_circle(c);
+  if (!roots) return -1;
+  const int n = (int)c->n;
+  const dim3 block(256);
+  if (dev_grad_tmpl_c64) {
+    hipLaunchKernelGGL(mfh::peak_grad_kernel<true>, dim3((n + 255) / 256, nt), block, 0, c->stream,
+                       (const cf *)dev_data_c64, data_stride, nd, dev_idx, (const cf *)dev_gval_c64, nd, nt, (int)nb,
+                       roots, n, (cf *)dev_grad_tmpl_c64, (long long)n);
+    if (!hip_ok(hipGetLastError(), "peak_grad_kernel")) return -1;
+  }
+  if (dev_grad_data_c64) {
+    hipLaunchKernelGGL(mfh::peak_grad_kernel<false>, dim3((n + 255) / 256, nd), block, 0, c->stream,
+                       (const cf *)dev_tmpl_c64, tmpl_stride, nt, dev_idx, (const cf *)dev_gval_c64, nd, nt, (int)nb,
+                       roots, n, (cf *)dev_grad_data_c64, (long long)n);
+    if (!hip_ok(hipGetLastError(), "peak_grad_kernel")) return -1;
+  }
+  return 0;
+}
+
+extern "C" int mfh_peak_values_async(mfh_ctx *ctx, const void *dev_data_c64, long long data_stride, int nd,
+                                     const void *dev_tmpl_c64, long long tmpl_stride, int nt, size_t nb,
+                                     const int64_t *dev_idx, void *dev_out_c64) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (nd < 1 || nt < 1 || nb < 1) return fail("empty peak array (%d x %d x %zu)", nd, nt, nb);
+  if (!dev_data_c64 || !dev_tmpl_c64 || !dev_idx || !dev_out_c64) return fail("null pointer");
+  if (data_stride < (long long)c->n || tmpl_stride < (long long)c->n) return fail("row stride below n");
+  hipSetDevice(c->dev);
+  const cf *roots = mfh::unit_circle(c);
+  if (!roots) return -1;
+  const long long npeaks = (long long)nd * nt * (long long)nb;
+  hipLaunchKernelGGL(mfh::peak_value_kernel, dim3((unsigned)((npeaks + 3) / 4)), dim3(256), 0, c->stream,
+                     (const cf *)dev_data_c64, data_stride, (const cf *)dev_tmpl_c64, tmpl_stride, nt, (int)nb, npeaks,
+                     dev_idx, roots, (int)c->n, (cf *)dev_out_c64);
+  return hip_ok(hipGetLastError(), "peak_value_kernel") ? 0 : -1;
+}

@@ -69,6 +69,11 @@ class HierarchicalFilter(MatchedFilter):
         self._prec = ("fp32", 0.0)
         self._prec_set = None
 
+    def _apply_tuning(self):
+        from . import autotune
+        autotune.apply(self._gpu, "flat", self.n, self.ndata, self.ntemplates)
+        autotune.apply(self._gpu, "hier", self.n, self.ndata, self.ntemplates)
+
     # ----------------------------------------------------------- settings
     def set_reference(self, power):
         """Reference output-power profile (length n, any scale), or None."""

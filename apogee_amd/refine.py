@@ -38,6 +38,8 @@ class RefineQueue:
         from .device import parse as _parse_device
         self.device = _parse_device(device)
         self._q = self._queue_factory(self.device.index)
+        from . import autotune
+        autotune.apply(self._q, "queue")
         self._staged = []          # add(): (data, templates, binsize, threshold, window)
         self._planned = None       # arrays of the current plan
         self._dirty = True

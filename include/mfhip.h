@@ -207,6 +207,29 @@ int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value);
 int mfh_prof_enable(mfh_ctx *ctx, int on);
 int mfh_prof_read(mfh_ctx *ctx, const char *name, double *total_ms, long *count);
 
+/* ---- Derivatives of peak values -----------------------------------------
+ * For reported peaks idx [nd][nt][nb] (lags; -1 = nothing reported) of data
+ * rows D and template rows T (device pointers, row strides in elements >= n,
+ * the context's n), and the incoming gradient gval [nd][nt][nb] of a real
+ * loss with respect to each value (dL/dRe z + i dL/dIm z), queue
+ *     grad_data[d][f] = sum_{t,b} gval T_t[f] exp(-2 pi i f k / n)
+ *     grad_tmpl[t][f] = sum_{d,b} conj(gval) D_d[f] exp(+2 pi i f k / n)
+ * into dense [nd][n] / [nt][n] device arrays (either may be null).  Needs only
+ * the stored lags: the correlation series is never rebuilt.
+ * Replaces: nothing in the reference (the peak-only filter there has no
+ * derivative); the forward it differentiates is mfh_flat_peaks. */
+int mfh_peak_grad_async(mfh_ctx *ctx, const void *dev_data_c64, long long data_stride, int nd,
+                        const void *dev_tmpl_c64, long long tmpl_stride, int nt, size_t nb,
+                        const int64_t *dev_idx, const void *dev_gval_c64, void *dev_grad_data_c64,
+                        void *dev_grad_tmpl_c64);
+
+/* z at stored lags: out[d][t][b] = sum_f D_d[f] conj(T_t[f]) exp(+2 pi i f k / n)
+ * (FP64 sums of FP32 products), 0 where idx is -1.  Recomputes a peak value
+ * without the series.  Replaces: nothing in the reference. */
+int mfh_peak_values_async(mfh_ctx *ctx, const void *dev_data_c64, long long data_stride, int nd,
+                          const void *dev_tmpl_c64, long long tmpl_stride, int nt, size_t nb,
+                          const int64_t *dev_idx, void *dev_out_c64);
+
 /* ---- Uneven batched refine over a device work queue ---------------------
  * Requests of mixed transform lengths (64 .. 65536), each one data spectrum
  * and c consecutive template spectra in a shared pool of complex64, are

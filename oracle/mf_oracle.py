@@ -312,3 +312,35 @@ def peak_value_grad(D, T, k):
     f = np.arange(n)
     w = np.exp(2j * np.pi * f * int(k) / n)
     return np.conj(T) * w, D * w
+
+
+def peak_grads(D, T, idx, g):
+    """Gradients of a real loss through reported peak values, FP64.
+
+    ``idx`` (nd, nt, nb) holds the stored lags (-1: nothing reported, a
+    constant 0), ``g`` (nd, nt, nb) the incoming gradient dL/dRe z + i dL/dIm z
+    of each value (PyTorch's convention for complex tensors).  With
+    w = exp(2 pi i / n) and z = sum_f D[f] conj(T[f]) w^(f k):
+        dL/dD_d[f] = sum_{t,b} g conj(dz/dD)        = sum g T_t[f] w^(-f k)
+        dL/dT_t[f] = sum_{d,b} conj(g) dz/dconj(T)  = sum conj(g) D_d[f] w^(f k)
+    (z is holomorphic in D and in conj(T)).  Returns (grad_D, grad_T).
+    """
+    D = np.asarray(D, dtype=np.complex128)
+    T = np.asarray(T, dtype=np.complex128)
+    idx = np.asarray(idx)
+    g = np.asarray(g, dtype=np.complex128)
+    nd, n = D.shape
+    nt = T.shape[0]
+    f = np.arange(n)
+    gD = np.zeros((nd, n), np.complex128)
+    gT = np.zeros((nt, n), np.complex128)
+    for d in range(nd):
+        for t in range(nt):
+            for b in range(idx.shape[2]):
+                k = int(idx[d, t, b])
+                if k < 0:
+                    continue
+                w = np.exp(2j * np.pi * ((f * k) % n) / n)
+                gD[d] += g[d, t, b] * T[t] * np.conj(w)
+                gT[t] += np.conj(g[d, t, b]) * D[d] * w
+    return gD, gT

@@ -20,6 +20,9 @@ class FakeContext:
         self.last_refinements = 0
         FakeContext.calls.append(("create", n, ndata, ntemplates))
 
+    def set_tuning(self, key, value):
+        FakeContext.calls.append(("tune", key, value))
+
     def set_rows(self, kind, row0, a):
         a = np.asarray(a)
         dst = self.D if kind == "data" else self.T
@@ -75,6 +78,9 @@ class FakeQueue:
         self.index = index
         self.pool = None
         self.plan_args = None
+
+    def set_tuning(self, key, value):
+        pass
 
     def set_pool(self, pool):
         self.pool = np.array(pool, dtype=np.complex64)

@@ -242,3 +242,39 @@ def test_run_series_contract(fake):
     np.testing.assert_array_equal(out["index"], want[0])
     with pytest.raises(ValueError, match="no data"):
         f.run()                                                    # series consumed the slots
+
+
+def test_autotune_table_lookup(tmp_path, monkeypatch):
+    """Nearest entry of the same operation and length, in log (ndata, ntemplates)."""
+    import json
+    from apogee_amd import autotune
+    table = {"entries": [
+        {"op": "flat", "n": 4096, "ndata": 16, "ntemplates": 1024, "best": {"dpw": 8}},
+        {"op": "flat", "n": 4096, "ndata": 64, "ntemplates": 256, "best": {"dpw": 2}},
+        {"op": "flat", "n": 1024, "ndata": 16, "ntemplates": 1024, "best": {"dpw": 16}},
+        {"op": "hier", "n": 4096, "ndata": 16, "ntemplates": 1024, "best": {"refine_grid": 512}},
+        {"op": "queue", "workload": "uneven", "best": {"grid_per_cu": 1}}]}
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps(table))
+    monkeypatch.setattr(autotune, "TABLE_PATH", str(path))
+    monkeypatch.setattr(autotune, "_table", None)
+    assert autotune.lookup("flat", 4096, 16, 2048) == {"dpw": 8}
+    assert autotune.lookup("flat", 4096, 48, 256) == {"dpw": 2}
+    assert autotune.lookup("flat", 1024, 1, 1) == {"dpw": 16}
+    assert autotune.lookup("flat", 2048, 16, 1024) in ({"dpw": 8}, {"dpw": 16})
+    assert autotune.lookup("hier", 4096, 16, 1024) == {"refine_grid": 512}
+    assert autotune.lookup("queue") == {"grid_per_cu": 1}
+    assert autotune.lookup("nothing") == {}
+
+    class Sink:
+        def __init__(self):
+            self.got = {}
+
+        def set_tuning(self, k, v):
+            self.got[k] = v
+    s = Sink()
+    autotune.apply(s, "flat", 4096, 16, 1024)
+    assert s.got == {"dpw": 8}
+    monkeypatch.setattr(autotune, "TABLE_PATH", str(tmp_path / "missing.json"))
+    monkeypatch.setattr(autotune, "_table", None)
+    assert autotune.lookup("flat", 4096, 16, 1024) == {}

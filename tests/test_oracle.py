@@ -226,6 +226,76 @@ def test_peak_value_gradient_against_finite_differences():
         assert abs(dz - (-1j) * b[f]) < 1e-6 * abs(b[f]) + 1e-9
 
 
+def _peak_loss(D, T, idx, c):
+    """A real loss of the peak values: sum Re(c z) + |z|^2 at the stored lags."""
+    n = D.shape[-1]
+    f = np.arange(n)
+    L = 0.0
+    for d, t, b in zip(*np.nonzero(idx >= 0)):
+        z = np.sum(D[d] * np.conj(T[t]) * np.exp(2j * np.pi * f * idx[d, t, b] / n))
+        L += (c[d, t, b] * z).real + abs(z) ** 2
+    return L
+
+
+def _loss_grad(D, T, idx, c):
+    """dL/dRe z + i dL/dIm z of _peak_loss: conj(c) + 2 z."""
+    z = np.zeros(idx.shape, np.complex128)
+    n = D.shape[-1]
+    f = np.arange(n)
+    for d, t, b in zip(*np.nonzero(idx >= 0)):
+        z[d, t, b] = np.sum(D[d] * np.conj(T[t]) * np.exp(2j * np.pi * f * idx[d, t, b] / n))
+    return np.where(idx >= 0, np.conj(c) + 2 * z, 0)
+
+
+def test_peak_grads_against_finite_differences():
+    """Real and imaginary parts of every input, central differences in FP64."""
+    rng = np.random.default_rng(21)
+    nd, nt, n, nb = 2, 3, 16, 2
+    D = cnoise(rng, (nd, n)).astype(np.complex128)
+    T = cnoise(rng, (nt, n)).astype(np.complex128)
+    idx = rng.integers(0, n, size=(nd, nt, nb))
+    idx[0, 1, 1] = -1                      # an empty bin contributes nothing
+    c = cnoise(rng, (nd, nt, nb)).astype(np.complex128)
+    gD, gT = O.peak_grads(D, T, idx, _loss_grad(D, T, idx, c))
+    h = 1e-6
+    for X, G, which in ((D, gD, "D"), (T, gT, "T")):
+        for r in range(X.shape[0]):
+            for f in (0, 5, 15):
+                for unit in (1.0, 1j):
+                    Xp, Xm = X.copy(), X.copy()
+                    Xp[r, f] += h * unit
+                    Xm[r, f] -= h * unit
+                    args_p = (Xp, T) if which == "D" else (D, Xp)
+                    args_m = (Xm, T) if which == "D" else (D, Xm)
+                    fd = (_peak_loss(*args_p, idx, c) - _peak_loss(*args_m, idx, c)) / (2 * h)
+                    got = G[r, f].real if unit == 1.0 else G[r, f].imag
+                    assert abs(fd - got) <= 1e-6 * (abs(got) + 1.0), (which, r, f, unit, fd, got)
+
+
+def test_peak_grads_against_torch_autograd():
+    """The same gradients from PyTorch's complex autograd through n*ifft."""
+    import torch
+    rng = np.random.default_rng(22)
+    nd, nt, n, nb = 3, 2, 64, 4
+    D = cnoise(rng, (nd, n)).astype(np.complex128)
+    T = cnoise(rng, (nt, n)).astype(np.complex128)
+    idx = rng.integers(0, n, size=(nd, nt, nb))
+    idx[2, 0, :2] = -1
+    c = cnoise(rng, (nd, nt, nb)).astype(np.complex128)
+    Dt = torch.tensor(D, requires_grad=True)
+    Tt = torch.tensor(T, requires_grad=True)
+    z = n * torch.fft.ifft(Dt[:, None, :] * torch.conj(Tt)[None, :, :], dim=-1)
+    it = torch.tensor(np.where(idx >= 0, idx, 0))
+    zk = torch.gather(z, 2, it)
+    m = torch.tensor(idx >= 0)
+    ct = torch.tensor(c)
+    L = (torch.where(m, (ct * zk).real + zk.abs() ** 2, torch.zeros((), dtype=torch.float64))).sum()
+    L.backward()
+    gD, gT = O.peak_grads(D, T, idx, _loss_grad(D, T, idx, c))
+    np.testing.assert_allclose(gD, Dt.grad.resolve_conj().numpy(), rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(gT, Tt.grad.resolve_conj().numpy(), rtol=1e-10, atol=1e-10)
+
+
 # --------------------------------------------- the FP32 CPU restatement
 
 @pytest.mark.parametrize("name", CASES)

@@ -316,8 +316,6 @@ class MatchedFilter:
         (nblocks, ntemplates, nbins) in caller order.  A later ``run()``
         requires ``set_data()`` again (python/matchedfilter/__init__.py:545-594).
         """
-        if self.n > 16384:
-            raise UnsupportedSize("run_series at n=%d is not built yet (n <= 16384 is)" % self.n)
         ser, layout, binsize, t0, nt = self._series_layout(series, starts, win_start, win_end,
                                                            binsize, templates)
         self._dataset = False

@@ -72,6 +72,88 @@ int launch_fwd(Ctx *c, const cf *series, size_t nseries, const long long *starts
   return hip_ok(hipGetLastError(), "series_forward_kernel") ? 0 : -1;
 }
 
+// Four-step forward transform for n = N1 x 256 (the flat filter's split,
+// f = f1 + N1 f2, k = k2 + 256 k1): columns take conj(x)/n along f2 through a
+// 256-point inverse FFT and the twiddle w_n^(f1 k2) into scratch Y[k2][f1];
+// rows run N1-point inverse FFTs over f1 and store conj(z) at k2 + 256 k1.
+template <int N1>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+series_big_cols(const cf *series, long long nseries, const long long *starts, const cf *tw256, const cf *rootH,
+                const cf *rootL, cf *Y) {
+  constexpr int N = N1 * 256;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int s = threadIdx.x & 15;
+  const int t = threadIdx.x >> 4;
+  cf *xch = reinterpret_cast<cf *>(smem) + s * FlatShape<256>::XCH;
+  const int b = blockIdx.y;
+  const long long s0 = starts[b];
+  const int f1 = blockIdx.x * 16 + s;
+  const float inv = 1.0f / (float)N;
+  cf v[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const long long k = s0 + f1 + N1 * (t + 16 * u);
+    const cf x = k < nseries ? series[k] : cf{0.f, 0.f};
+    v[u] = cf{x.x * inv, -(x.y * inv)};
+  }
+  inverse_fft<256>(v, t, xch, tw256);
+  cf *Yb = Y + (long long)b * N;
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const int k2 = t + 16 * u;
+    const int m = f1 * k2;
+    Yb[(long long)k2 * N1 + f1] = cmul(v[u], cmul(rootH[m >> 8], rootL[m & 255]));
+  }
+}
+
+template <int N1>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+series_big_rows(const cf *Y, const cf *twN1, cf *out) {
+  constexpr int N = N1 * 256;
+  using S = FlatShape<N1>;
+  constexpr int TPT = S::TPT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int slot = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  cf *xch = reinterpret_cast<cf *>(smem) + slot * S::XCH;
+  const int b = blockIdx.y;
+  const int k2 = blockIdx.x * S::SLOTS + slot;
+  const cf *Yb = Y + (long long)b * N;
+  cf v[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) v[u] = Yb[(long long)k2 * N1 + t + TPT * u];
+  inverse_fft<N1>(v, t, xch, twN1);
+  cf *ob = out + (long long)b * N;
+#pragma unroll
+  for (int u = 0; u < 16; u++) ob[k2 + 256 * (t + TPT * u)] = cf{v[u].x, -v[u].y};
+}
+
+template <int N1>
+int launch_fwd_big(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out) {
+  constexpr int N = N1 * 256;
+  using S = FlatShape<N1>;
+  const cf *tw256 = c->twiddles(256);
+  const cf *twN1 = c->twiddles(N1);
+  const cf *roots = big_roots(c, N1);
+  if (!tw256 || !twN1 || !roots) return -1;
+  // scratch for at most 256 MiB of blocks at a time (and 65535 per launch)
+  const int chunk = (int)std::max<long long>(1, std::min<long long>({(long long)nblocks, 65535ll,
+                                                                     (256ll << 20) / ((long long)N * 8)}));
+  cf *Y = (cf *)c->scratch("series_big_y", (size_t)chunk * N * sizeof(cf));
+  if (!Y) return -1;
+  const size_t lds_cols = 16 * FlatShape<256>::XCH * sizeof(cf);
+  const size_t lds_rows = (size_t)S::SLOTS * S::XCH * sizeof(cf);
+  Ctx::ProfScope ps(c, "series_forward");
+  for (int b0 = 0; b0 < nblocks; b0 += chunk) {
+    const int nb = std::min(chunk, nblocks - b0);
+    hipLaunchKernelGGL(series_big_cols<N1>, dim3(N1 / 16, nb), dim3(256), lds_cols, c->stream, series,
+                       (long long)nseries, starts + b0, tw256, roots, roots + N1, Y);
+    hipLaunchKernelGGL(series_big_rows<N1>, dim3(256 / S::SLOTS, nb), dim3(256), lds_rows, c->stream, Y, twN1,
+                       out + (long long)b0 * N);
+  }
+  return hip_ok(hipGetLastError(), "series_big") ? 0 : -1;
+}
+
 }  // namespace
 
 int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out) {
@@ -85,6 +167,8 @@ int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long l
     case 4096: return launch_fwd<4096>(c, series, nseries, starts, nblocks, out);
     case 8192: return launch_fwd<8192>(c, series, nseries, starts, nblocks, out);
     case 16384: return launch_fwd<16384>(c, series, nseries, starts, nblocks, out);
+    case 32768: return launch_fwd_big<128>(c, series, nseries, starts, nblocks, out);
+    case 65536: return launch_fwd_big<256>(c, series, nseries, starts, nblocks, out);
     default: return fail("transform length %zu is not supported", c->n);
   }
 }

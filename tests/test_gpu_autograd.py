@@ -6,7 +6,7 @@
 * The differentiable peak fit: parametric chirp templates theta = (A, t0, a,
   b), injections of perturbed theta at SNR 10 in unit noise, loss
   -sum |z_peak|^2 / sigma^2 over 512-lag bins.  d loss / d theta against FP64
-  central finite differences (step 1e-4) and against a torch.fft autograd
+  central finite differences (step 1e-4, one Richardson step) and against a torch.fft autograd
   reference in complex128, both at the stored lags, to 1e-3 relative.
 * The forward equals MatchedFilter.run; recomputed values equal the forward.
 """
@@ -38,7 +38,7 @@ def test_backward_matches_the_fp64_oracle(gpu_device):
     pf = PeakFilter(n, nd, nt, device="gpu")
     Dg = torch.from_numpy(D).cuda().requires_grad_(True)
     Tg = torch.from_numpy(T).cuda().requires_grad_(True)
-    val, idx = pf(Dg, Tg, binsize=bs, threshold=34.0)     # some bins stay empty
+    val, idx = pf(Dg, Tg, binsize=bs, threshold=73.0)     # about half the bins stay empty
     idx_h = idx.cpu().numpy()
     assert (idx_h < 0).any() and (idx_h >= 0).any()
     g = cnoise(rng, val.shape)
@@ -154,16 +154,22 @@ def test_peak_fit_gradients(gpu_device):
         rel = np.linalg.norm(g[:, p] - g_r[:, p]) / np.linalg.norm(g_r[:, p])
         assert rel <= 1e-3, (name, rel)
 
-    # FP64 central differences, step 1e-4, at the stored lags
+    # FP64 central differences at the stored lags, base step 1e-4 with one
+    # Richardson step (h and h/2): b multiplies f up to 1500, so a plain
+    # central difference at 1e-4 keeps an O((1500 h)^2) = 0.4% bias
     h = 1e-4
     base = theta.astype(np.float64)
-    for t in (0, 7, 19):
+
+    def central(t, p, step):
         sl = slice(t, t + 1)
+        tp, tm = base[sl].copy(), base[sl].copy()
+        tp[0, p] += step
+        tm[0, p] -= step
+        return (loss_at_lags_np(tp, D, idx_h[:, sl], n) - loss_at_lags_np(tm, D, idx_h[:, sl], n)) / (2 * step)
+
+    for t in (0, 7, 19):
         for p in (1, 2, 3):
-            tp, tm = base[sl].copy(), base[sl].copy()
-            tp[0, p] += h
-            tm[0, p] -= h
-            fd = (loss_at_lags_np(tp, D, idx_h[:, sl], n) - loss_at_lags_np(tm, D, idx_h[:, sl], n)) / (2 * h)
+            fd = (4 * central(t, p, h / 2) - central(t, p, h)) / 3
             scale = np.abs(g_r[:, p]).max()
-            assert abs(g[t, p] - fd) <= 1e-3 * abs(fd) + 1e-4 * scale, (t, p, g[t, p], fd)
-            assert abs(g_r[t, p] - fd) <= 1e-6 * abs(fd) + 1e-7 * scale
+            assert abs(g[t, p] - fd) <= 1e-3 * abs(fd) + 1e-5 * scale, (t, p, g[t, p], fd)
+            assert abs(g_r[t, p] - fd) <= 1e-5 * abs(fd) + 1e-7 * scale, (t, p, g_r[t, p], fd)

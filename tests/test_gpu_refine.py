@@ -89,7 +89,7 @@ def test_ragged_bins_windows_thresholds_and_pieces(gpu_device):
         (65536, 1, 16, 0.0, None),          # 4096 bins: window pieces of the four-step
         (65536, 2, 1000, 4.0, (123, 65000)), (65536, 1, 65536, 0.0, (0, 65536)),
         (1024, 0, 64, 0.0, None),           # no candidates
-        (4096, 1, 64, 50.0, None),          # nothing crosses
+        (4096, 1, 64, 1000.0, None),        # nothing crosses (|z| ~ 64 on this noise)
     ]
     reqs = [(cnoise(rng, n), cnoise(rng, (c, n)), bs, thr, w) for n, c, bs, thr, w in cases]
     q = mf.RefineQueue()
@@ -163,7 +163,9 @@ def test_device_pool_async_outputs_and_in_place_updates(gpu_device):
         np.testing.assert_array_equal(got_i[o:o + k], ref[r]["index"].reshape(-1))
         np.testing.assert_array_equal(got_v[o:o + k], ref[r]["value"].reshape(-1))
     # the plan reads the pool in place: new contents, same plan, same graph
-    pool.mul_(-1.0)
+    # (negating the data spectra negates every peak value, same lags)
+    for d0, m in zip(doff, n):
+        pool[int(d0):int(d0 + m)].mul_(-1.0)
     torch.cuda.synchronize()
     flipped = q.run()
     for a, b in zip(ref, flipped):

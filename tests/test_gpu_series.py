@@ -34,11 +34,11 @@ def check_against_oracle(series, starts, ws, we, T, binsize, threshold, got):
                            threshold, (int(ws[b]), int(we[b])), context="block %d" % b)
 
 
-@pytest.mark.parametrize("n", [256, 1024, 4096])
+@pytest.mark.parametrize("n", [256, 1024, 4096, 16384, 32768, 65536])
 def test_half_overlap_real_series(n, gpu_device):
     """A real float32 series with 50% overlap (starts = b n/2), window [n/4, 3n/4)."""
     rng = np.random.default_rng(n)
-    nt, nblk = 5, 12
+    nt, nblk = (5, 12) if n <= 4096 else (3, 5)
     series = rng.standard_normal(nblk * n // 2 + n // 2).astype(np.float32)
     T = (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64)
     starts = (np.arange(nblk) * (n // 2)).astype(np.uintp)

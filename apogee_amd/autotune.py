@@ -135,7 +135,7 @@ def _tune_hier(device, n, nd, nt, binsize, band, reps, rng):
     ms = {}
     # a gate that passes about 1% of noise pairs: the refine launch sees a
     # short, uneven list, which is what its grid size trades against
-    cmax = np.sort(ctx.hier_coarse_max(0, nd, 0, nt, 0, band, 0).ravel())
+    cmax = np.sort(ctx.hier_coarse_max(0, nd, 0, nt, 0, n, 0).ravel())
     gate = float(cmax[int(0.99 * cmax.size)])
     for grid in CANDIDATES["hier"]["refine_grid"]:
         ctx.set_tuning("refine_grid", grid)

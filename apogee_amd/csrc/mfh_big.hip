@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
   extern __shared__ __align__(16) unsigned char smem[];
   const int s = threadIdx.x & 15;
   const int t = threadIdx.x >> 4;
-  cf *xch = reinterpret_cast<cf *>(smem) + s * S256::XCH;
+  cf *xch = reinterpret_cast<cf *>(smem) + s * kColStride;
   const long long pl = blockIdx.y;  // pair within the chunk
   int d = 0, tt = 0;
   const bool valid = big_pair(a, a.p0 + pl, d, tt);  // uniform across the workgroup
@@ -103,11 +103,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
   constexpr int SLOTS = S::SLOTS;
   extern __shared__ __align__(16) unsigned char smem[];
   cf *xch_all = reinterpret_cast<cf *>(smem);
-  uint64_t *bkey = reinterpret_cast<uint64_t *>(xch_all + SLOTS * S::XCH);
+  uint64_t *bkey = reinterpret_cast<uint64_t *>(xch_all + SLOTS * S::XSTRIDE);
   cf *bval = reinterpret_cast<cf *>(bkey + a.nb);
   const int slot = threadIdx.x / TPT;
   const int t = threadIdx.x % TPT;
-  cf *xch = xch_all + slot * S::XCH;
+  cf *xch = xch_all + slot * S::XSTRIDE;
   const long long pl = blockIdx.x;
   int d = 0, tt = 0;
   if (!big_pair(a, a.p0 + pl, d, tt)) return;  // uniform; before any barrier
@@ -190,7 +190,7 @@ constexpr size_t kLds = 160 * 1024;
 template <int N1>
 size_t rows_max_bins() {
   using S = FlatShape<N1>;
-  return (kLds - (size_t)S::SLOTS * S::XCH * sizeof(cf)) / (sizeof(uint64_t) + sizeof(cf));
+  return (kLds - (size_t)S::SLOTS * S::XSTRIDE * sizeof(cf)) / (sizeof(uint64_t) + sizeof(cf));
 }
 
 template <int N1>
@@ -219,7 +219,7 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
   cf *scr = (cf *)c->scratch("big_scratch", (size_t)chunk * N * sizeof(cf));
   if (!scr) return -1;
   a.scratch = scr;
-  const size_t lds_cols = 16 * FlatShape<256>::XCH * sizeof(cf);
+  const size_t lds_cols = 16 * kColStride * sizeof(cf);
   const float thr = r.threshold;
   a.seed = thr > 0.f ? (float)((double)thr * (double)thr) : 0.f;
   a.binsize = (uint32_t)r.binsize;
@@ -254,7 +254,7 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
         a.ostride = (uint32_t)nb_total;
         a.out_idx = r.out_idx + b0;
         a.out_val = r.out_val + b0;
-        const size_t lds = (size_t)S::SLOTS * S::XCH * sizeof(cf) + cnt * (sizeof(uint64_t) + sizeof(cf));
+        const size_t lds = (size_t)S::SLOTS * S::XSTRIDE * sizeof(cf) + cnt * (sizeof(uint64_t) + sizeof(cf));
         hipLaunchKernelGGL(big_rows_kernel<N1>, dim3((unsigned)np), dim3(256), lds, c->stream, a);
         if (!hip_ok(hipGetLastError(), "big_rows_kernel")) return -1;
       }

@@ -59,7 +59,17 @@ struct FlatShape {
   static constexpr int FIRST = 1 << (LOGN % 4);  // radix of the first pass (1: none)
   static constexpr int NPASS16 = LOGN / 4;
   static constexpr int XCH = N + N / 16;          // padded exchange elements per slot
+  // distance between slots' exchange buffers: one element more when several
+  // slots share a wave (TPT < 64), so that equal offsets of neighbouring
+  // slots fall in different LDS banks
+  static constexpr int XSTRIDE = TPT < 64 ? XCH + 1 : XCH;
 };
+
+// Four-step column transforms: 16 lanes of a row group serve 16 adjacent
+// columns, each column a 256-point transform with its own exchange buffer.
+// A buffer stride of 273 elements (546 dwords, 34 mod 64) puts the 16 lanes'
+// equal offsets in 16 distinct bank pairs; 272 would fold them onto two.
+constexpr int kColStride = FlatShape<256>::XCH + 1;
 
 __device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
 
@@ -244,12 +254,12 @@ flat_kernel(FlatArgs a) {
   constexpr int TPT = S::TPT;
   extern __shared__ __align__(16) unsigned char smem[];
   cf *xch_all = reinterpret_cast<cf *>(smem);
-  uint64_t *bins_all = reinterpret_cast<uint64_t *>(xch_all + S::SLOTS * S::XCH);
+  uint64_t *bins_all = reinterpret_cast<uint64_t *>(xch_all + S::SLOTS * S::XSTRIDE);
 
   const int slot = threadIdx.x / TPT;
   const int t0 = threadIdx.x % TPT;
   const int lane = threadIdx.x & 63;
-  cf *xch = xch_all + slot * S::XCH;
+  cf *xch = xch_all + slot * S::XSTRIDE;
   const int nb = (int)a.nb;
 
   // Tiled mode: this slot's template stays in registers while the workgroup

@@ -18,13 +18,13 @@ constexpr size_t kLdsLimit = 160 * 1024;
 template <int N>
 size_t flat_lds_bytes(size_t nb) {
   using S = FlatShape<N>;
-  return (size_t)S::SLOTS * S::XCH * sizeof(cf) + 2 * (size_t)S::SLOTS * nb * sizeof(uint64_t);
+  return (size_t)S::SLOTS * S::XSTRIDE * sizeof(cf) + 2 * (size_t)S::SLOTS * nb * sizeof(uint64_t);
 }
 
 template <int N>
 size_t flat_max_bins() {
   using S = FlatShape<N>;
-  const size_t fixed = (size_t)S::SLOTS * S::XCH * sizeof(cf);
+  const size_t fixed = (size_t)S::SLOTS * S::XSTRIDE * sizeof(cf);
   return (kLdsLimit - fixed) / (2 * (size_t)S::SLOTS * sizeof(uint64_t));
 }
 

@@ -49,10 +49,19 @@ struct QTile {
 
 constexpr int kQThreads = 1024;
 constexpr int kQMaxLog2 = 16;
-// exchange space: 1024 lanes x 16 points, padded by 1/16, at every length
-constexpr size_t kQXch = (size_t)kQThreads * 17 * sizeof(cf);
-// per-bin (key, value) table of a four-step pair, after the exchange space
-constexpr int kQBigBins = 1535;
+// exchange space of a tile: 1024 lanes x 16 points, padded by 1/16, plus one
+// element per slot where slots share a wave (largest at n = 64: 256 slots)
+template <int N>
+constexpr size_t q_xch_bytes() {
+  return (size_t)(kQThreads / FlatShape<N>::TPT) * FlatShape<N>::XSTRIDE * sizeof(cf);
+}
+constexpr size_t kQXch = q_xch_bytes<64>();
+static_assert(q_xch_bytes<128>() <= kQXch && q_xch_bytes<4096>() <= kQXch && q_xch_bytes<16384>() <= kQXch, "");
+static_assert((size_t)64 * kColStride * sizeof(cf) <= kQXch, "four-step columns");
+static_assert(q_xch_bytes<128>() <= kQXch && q_xch_bytes<256>() <= kQXch, "four-step rows");
+// per-bin (key, value) table of a four-step pair, after the exchange space;
+// with the 16-byte tile slot the total is the 160 KiB a workgroup may hold
+constexpr int kQBigBins = (int)((160 * 1024 - 16 - kQXch) / (sizeof(uint64_t) + sizeof(cf)));
 constexpr size_t kQLds = kQXch + (size_t)kQBigBins * (sizeof(uint64_t) + sizeof(cf));
 
 struct QArgs {
@@ -125,7 +134,7 @@ __device__ __forceinline__ void q_select(const cf *v, int t, int tid, const QIte
 template <int N>
 __device__ __forceinline__ void q_flat(const QArgs &a, const QTile &tile, cf *smem, int tid) {
   constexpr int TPT = N / 16;
-  constexpr int XCH = N + N / 16;
+  constexpr int XCH = FlatShape<N>::XSTRIDE;
   const int slot = tid / TPT;
   const int t = tid % TPT;
   const bool valid = slot < tile.count;
@@ -189,7 +198,7 @@ __device__ __forceinline__ void q_big(const QArgs &a, const QTile &tile, cf *sme
     const int s = tid & 15;
     const int t = (tid >> 4) & 15;
     const int g = tid >> 8;
-    cf *xch = smem + (g * 16 + s) * FlatShape<256>::XCH;
+    cf *xch = smem + (g * 16 + s) * kColStride;
     for (int r = 0; r < N1 / 64; r++) {
       const int f1 = (r * 4 + g) * 16 + s;
       cf v[16];
@@ -213,7 +222,7 @@ __device__ __forceinline__ void q_big(const QArgs &a, const QTile &tile, cf *sme
   constexpr int SLOTS = kQThreads / TPT;
   const int slot = tid / TPT;
   const int t = tid % TPT;
-  cf *xch = smem + slot * FlatShape<N1>::XCH;
+  cf *xch = smem + slot * FlatShape<N1>::XSTRIDE;
   uint64_t *bkey = reinterpret_cast<uint64_t *>(reinterpret_cast<unsigned char *>(smem) + kQXch);
   cf *bval = reinterpret_cast<cf *>(bkey + kQBigBins);
   const int nb = (int)it.nb;

@@ -44,7 +44,7 @@ series_forward_kernel(const cf *series, long long nseries, const long long *star
     const cf x = (valid && k < nseries) ? series[k] : cf{0.f, 0.f};
     v[u] = cf{x.x * inv, -(x.y * inv)};
   }
-  inverse_fft<N>(v, t, xch_all + slot * S::XCH, tw);
+  inverse_fft<N>(v, t, xch_all + slot * S::XSTRIDE, tw);
   if (valid) {
     cf *o = out + (long long)b * N;
 #pragma unroll
@@ -57,7 +57,7 @@ int launch_fwd(Ctx *c, const cf *series, size_t nseries, const long long *starts
   using S = FlatShape<N>;
   const cf *tw = c->twiddles(N);
   if (!tw) return -1;
-  const size_t lds = (size_t)S::SLOTS * S::XCH * sizeof(cf);
+  const size_t lds = (size_t)S::SLOTS * S::XSTRIDE * sizeof(cf);
   static bool lds_set = false;
   if (!lds_set) {
     if (!hip_ok(hipFuncSetAttribute((const void *)series_forward_kernel<N>,
@@ -84,7 +84,7 @@ series_big_cols(const cf *series, long long nseries, const long long *starts, co
   extern __shared__ __align__(16) unsigned char smem[];
   const int s = threadIdx.x & 15;
   const int t = threadIdx.x >> 4;
-  cf *xch = reinterpret_cast<cf *>(smem) + s * FlatShape<256>::XCH;
+  cf *xch = reinterpret_cast<cf *>(smem) + s * kColStride;
   const int b = blockIdx.y;
   const long long s0 = starts[b];
   const int f1 = blockIdx.x * 16 + s;
@@ -115,7 +115,7 @@ series_big_rows(const cf *Y, const cf *twN1, cf *out) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int slot = threadIdx.x / TPT;
   const int t = threadIdx.x % TPT;
-  cf *xch = reinterpret_cast<cf *>(smem) + slot * S::XCH;
+  cf *xch = reinterpret_cast<cf *>(smem) + slot * S::XSTRIDE;
   const int b = blockIdx.y;
   const int k2 = blockIdx.x * S::SLOTS + slot;
   const cf *Yb = Y + (long long)b * N;
@@ -141,8 +141,8 @@ int launch_fwd_big(Ctx *c, const cf *series, size_t nseries, const long long *st
                                                                      (256ll << 20) / ((long long)N * 8)}));
   cf *Y = (cf *)c->scratch("series_big_y", (size_t)chunk * N * sizeof(cf));
   if (!Y) return -1;
-  const size_t lds_cols = 16 * FlatShape<256>::XCH * sizeof(cf);
-  const size_t lds_rows = (size_t)S::SLOTS * S::XCH * sizeof(cf);
+  const size_t lds_cols = 16 * kColStride * sizeof(cf);
+  const size_t lds_rows = (size_t)S::SLOTS * S::XSTRIDE * sizeof(cf);
   Ctx::ProfScope ps(c, "series_forward");
   for (int b0 = 0; b0 < nblocks; b0 += chunk) {
     const int nb = std::min(chunk, nblocks - b0);

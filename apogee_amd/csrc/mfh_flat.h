@@ -96,6 +96,65 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
   return v;
 }
 
+// Per-bin maxima of a wave's keys when the lag window is the whole transform
+// and bins are exactly 2 TPT lags wide: thread t's lags t + u*TPT, u < 16,
+// form 8 runs of two consecutive u, and each run lies in one bin for all 64
+// lanes of the wave.  Instead of one 64-lane reduction per run (6 exchanges
+// each), a transposing butterfly halves every lane's list of run maxima per
+// exchange step -- at xor distance 32 the upper lanes keep the upper half of
+// the runs, at 16 the upper quarter of theirs, at 8 one run -- and three
+// plain steps finish it: 10 exchanges instead of 48.  One lane per run then
+// raises the bin's key in LDS.
+template <int TPT>
+__device__ __forceinline__ uint64_t pair_run_max(const cf *v, int t, int r, float seed) {
+  const uint64_t a = make_key(v[2 * r], (uint32_t)(t + 2 * r * TPT), seed);
+  const uint64_t b = make_key(v[2 * r + 1], (uint32_t)(t + (2 * r + 1) * TPT), seed);
+  return a > b ? a : b;
+}
+
+__device__ __forceinline__ uint64_t xchg_max(uint64_t keep, uint64_t send, int m) {
+  const uint64_t got = __shfl_xor(send, m, 64);
+  return got > keep ? got : keep;
+}
+
+// R = 8 runs of two u each (bins of exactly 2 TPT lags), streamed so that at
+// most four partial maxima are live at once.
+template <int TPT>
+__device__ __forceinline__ void runs8_max_to_bins(const cf *v, int t, int lane, float seed, int bshift,
+                                                  uint64_t *bins) {
+  const bool up1 = (lane & 32) != 0, up2 = (lane & 16) != 0, up3 = (lane & 8) != 0;
+  uint64_t q[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint64_t lo = pair_run_max<TPT>(v, t, i, seed);
+    const uint64_t hi = pair_run_max<TPT>(v, t, i + 4, seed);
+    q[i] = xchg_max(up1 ? hi : lo, up1 ? lo : hi, 32);
+  }
+  const uint64_t r0 = xchg_max(up2 ? q[2] : q[0], up2 ? q[0] : q[2], 16);
+  const uint64_t r1 = xchg_max(up2 ? q[3] : q[1], up2 ? q[1] : q[3], 16);
+  uint64_t w = xchg_max(up3 ? r1 : r0, up3 ? r0 : r1, 8);
+  w = xchg_max(w, w, 4);
+  w = xchg_max(w, w, 2);
+  w = xchg_max(w, w, 1);
+  const int rho = (up1 ? 4 : 0) + (up2 ? 2 : 0) + (up3 ? 1 : 0);
+  if ((lane & 7) == 0 && w) {
+    const uint32_t b = ((uint32_t)(t - lane) + (uint32_t)(rho * 2 * TPT)) >> bshift;
+    atomicMax((unsigned long long *)&bins[b], (unsigned long long)w);
+  }
+}
+
+// Whole-transform window with bins of exactly 2 TPT lags (n/8 bins); other
+// shapes take the per-run reductions.  Returns false when the shape does not
+// qualify.
+template <int TPT>
+__device__ __forceinline__ bool full_window_bins(const cf *v, int t, int lane, float seed, int bshift,
+                                                 uint64_t *bins) {
+  constexpr int LT = ilog2(TPT);
+  if (bshift - LT != 1) return false;
+  runs8_max_to_bins<TPT>(v, t, lane, seed, bshift, bins);
+  return true;
+}
+
 // Twiddle tables, one per twiddled radix-16 pass, holding only the powers
 // r = 1, 2, 4, 8 of each butterfly's twiddle, laid out [row][k]:
 //   tw[pass_offset(p) + row*NS + k] = exp(+2 pi i (1 << row) k / (16 NS)),
@@ -348,40 +407,46 @@ flat_kernel(FlatArgs a) {
       }
     } else {
       // ---- several bins: per-thread runs, wave reductions, LDS max ----------
+      // whole-transform window with power-of-two bins of >= 64 lags: the
+      // transposing butterfly (wave-uniform test)
+      const bool full = MODE == kAligned && a.ws == 0 && a.we == (uint32_t)N && a.bshift >= 6;
       if constexpr (TPT >= 64) {
-        uint64_t acc = 0ull;
-        int acc_bin = -1;
+        if (full && full_window_bins<TPT>(v, t, lane, a.seed, a.bshift, bins)) {
+        } else {
+          uint64_t acc = 0ull;
+          int acc_bin = -1;
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-          const uint32_t k = (uint32_t)(t + u * TPT);
-          const uint64_t key = (k >= a.ws && k < a.we) ? make_key(v[u], k, a.seed) : 0ull;
-          bool uni;
-          uint32_t kf = 0;
-          if constexpr (MODE == kAligned) {
-            kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
-            uni = kf >= a.ws && kf < a.we;
-          } else {
-            kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
-            uni = kf >= a.ws && kf + 63 < a.we && bin_of(kf, a) == bin_of(kf + 63, a);
-          }
-          if (uni) {
-            const int b = (int)bin_of(kf, a);
-            if (b != acc_bin) {
-              if (acc_bin >= 0) {
-                const uint64_t w = wave_max_u64(acc);
-                if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
-              }
-              acc = 0ull;
-              acc_bin = b;
+          for (int u = 0; u < 16; u++) {
+            const uint32_t k = (uint32_t)(t + u * TPT);
+            const uint64_t key = (k >= a.ws && k < a.we) ? make_key(v[u], k, a.seed) : 0ull;
+            bool uni;
+            uint32_t kf = 0;
+            if constexpr (MODE == kAligned) {
+              kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
+              uni = kf >= a.ws && kf < a.we;
+            } else {
+              kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
+              uni = kf >= a.ws && kf + 63 < a.we && bin_of(kf, a) == bin_of(kf + 63, a);
             }
-            acc = key > acc ? key : acc;
-          } else if (key) {
-            atomicMax((unsigned long long *)&bins[bin_of(k, a)], (unsigned long long)key);
+            if (uni) {
+              const int b = (int)bin_of(kf, a);
+              if (b != acc_bin) {
+                if (acc_bin >= 0) {
+                  const uint64_t w = wave_max_u64(acc);
+                  if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
+                }
+                acc = 0ull;
+                acc_bin = b;
+              }
+              acc = key > acc ? key : acc;
+            } else if (key) {
+              atomicMax((unsigned long long *)&bins[bin_of(k, a)], (unsigned long long)key);
+            }
           }
-        }
-        if (acc_bin >= 0) {
-          const uint64_t w = wave_max_u64(acc);
-          if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
+          if (acc_bin >= 0) {
+            const uint64_t w = wave_max_u64(acc);
+            if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
+          }
         }
       } else {
         // several transforms share a wave: per-thread runs, then LDS atomics

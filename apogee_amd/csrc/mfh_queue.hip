@@ -91,6 +91,13 @@ __device__ __forceinline__ uint32_t q_bin(uint32_t k, const QItem &it) {
 template <int TPT>
 __device__ __forceinline__ void q_select(const cf *v, int t, int tid, const QItem &it, uint64_t *bkey) {
   const int lane = tid & 63;
+  if constexpr (TPT >= 64) {
+    // whole-transform window in n/8 bins: the transposing butterfly
+    if (it.ws == 0 && it.we == 16u * TPT && it.bshift == ilog2(TPT) + 1) {
+      runs8_max_to_bins<TPT>(v, t, lane, it.seed, it.bshift, bkey);
+      return;
+    }
+  }
   uint64_t acc = 0ull;
   int acc_bin = -1;
 #pragma unroll

@@ -40,6 +40,9 @@ struct BigArgs {
   cf *scratch;          // [chunk pair][256][N1]
   const cf *tw256, *twN1;  // Stockham tables of the two lengths
   const cf *rootH, *rootL; // w_N1^h (h < N1) and w_n^l (l < 256)
+  int nsplit;               // rows workgroups per pair (gridDim.y of the rows kernel)
+  uint64_t *pkey;           // nsplit > 1: per-part bin tables [pair][part][nb]
+  cf *pval;
   int64_t *out_idx;
   cf *out_val;
   uint32_t ws, we, binsize, nb, ostride;
@@ -71,10 +74,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
   const int s = threadIdx.x & 15;
   const int t = threadIdx.x >> 4;
   cf *xch = reinterpret_cast<cf *>(smem) + s * kColStride;
+  // the two root tables, copied to LDS: the twiddle gathers below are
+  // scattered, and LDS serves them far faster than the vector cache
+  cf *rH = reinterpret_cast<cf *>(smem) + 16 * kColStride;
+  cf *rL = rH + N1;
   const long long pl = blockIdx.y;  // pair within the chunk
   int d = 0, tt = 0;
   const bool valid = big_pair(a, a.p0 + pl, d, tt);  // uniform across the workgroup
   if (!valid) return;                                 // no barrier has run yet
+  for (int i = threadIdx.x; i < N1 + 256; i += 256) rH[i] = i < N1 ? a.rootH[i] : a.rootL[i - N1];
   const int f1 = blockIdx.x * 16 + s;
   const cf *D = a.data + (long long)d * a.data_stride;
   const cf *T = a.tmpl + (long long)tt * a.tmpl_stride;
@@ -84,18 +92,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
     const int f = f1 + N1 * (t + 16 * u);
     v[u] = cmul_conj(D[f], T[f]);
   }
-  inverse_fft<256>(v, t, xch, a.tw256);
+  inverse_fft<256>(v, t, xch, a.tw256);  // its barriers order the table copy before the reads
   cf *Y = a.scratch + pl * N;
 #pragma unroll
   for (int u = 0; u < 16; u++) {
     const int k2 = t + 16 * u;
     const int m = f1 * k2;                         // < N
-    const cf w = cmul(a.rootH[m >> 8], a.rootL[m & 255]);
+    const cf w = cmul(rH[m >> 8], rL[m & 255]);
     Y[(long long)k2 * N1 + f1] = cmul(v[u], w);
   }
 }
 
-// Rows: one workgroup per pair, SLOTS rows (k2) at a time.
+// Rows: nsplit workgroups per pair (blockIdx.y = part), each walking its
+// share of the 256 rows k2, SLOTS rows at a time.  With one part the table is
+// the result; with several, each part leaves its (key, value) table for
+// big_rows_finish_kernel, which keeps the largest key per bin.
 template <int N1>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) big_rows_kernel(BigArgs a) {
   using S = FlatShape<N1>;
@@ -115,19 +126,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
   for (int b = threadIdx.x; b < nb; b += 256) bkey[b] = 0ull;
   __syncthreads();
   const cf *Y = a.scratch + pl * (long long)(N1 * 256);
+  constexpr int G = 256 / SLOTS;  // row groups of the pair
+  const int gbeg = (int)blockIdx.y * G / a.nsplit, gend = ((int)blockIdx.y + 1) * G / a.nsplit;
+  const bool full8 = a.ws == 0 && a.we == (uint32_t)(N1 * 256) && a.bshift == ilog2(N1 * 256 / 8);
   FlatArgs fa;  // for bin_of / make_key
   fa.ws = a.ws;
   fa.we = a.we;
   fa.binsize = a.binsize;
   fa.bshift = a.bshift;
   fa.inv_bs = a.inv_bs;
-  for (int g = 0; g < 256 / SLOTS; g++) {
+  for (int g = gbeg; g < gend; g++) {
     const int k2 = g * SLOTS + slot;
     cf v[16];
 #pragma unroll
     for (int u = 0; u < 16; u++) v[u] = Y[(long long)k2 * N1 + t + TPT * u];
     inverse_fft<N1>(v, t, xch, a.twN1);
     // thread t holds z[k2 + 256 (t + TPT u)]; lags rise with u
+    if (full8) {
+      // n/8 bins over the whole transform: every wave's 64 lanes put u and
+      // u + 1 = 2 rho, 2 rho + 1 in bin rho (their lags span < 256 TPT)
+      const int lane = threadIdx.x & 63;
+      const int k2w = __builtin_amdgcn_readfirstlane(k2);
+      runs8_max_to_bins<256 * TPT>(v, (uint32_t)(k2 + 256 * t), (uint32_t)k2w, lane, a.seed, a.bshift, bkey);
+    } else {
     uint64_t acc = 0ull;
     int acc_bin = -1;
 #pragma unroll
@@ -145,6 +166,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
       }
     }
     if (acc) atomicMax((unsigned long long *)&bkey[acc_bin], (unsigned long long)acc);
+    }
     __syncthreads();
     // the holder of each bin's key records its value
 #pragma unroll
@@ -160,6 +182,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
     }
     __syncthreads();  // values settled before the next group raises keys; xch reusable
   }
+  if (a.nsplit > 1) {
+    const long long part = (pl * a.nsplit + blockIdx.y) * (long long)nb;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+      const uint64_t key = bkey[b];
+      a.pkey[part + b] = key;
+      a.pval[part + b] = key ? bval[b] : cf{0.f, 0.f};
+    }
+    return;
+  }
   const long long o = a.list_ooff ? a.list_ooff[a.p0 + pl] : ((long long)d * a.nt + tt) * a.ostride;
   int64_t *oidx = a.out_idx + o;
   cf *oval = a.out_val + o;
@@ -173,6 +204,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
       oval[b] = cf{0.f, 0.f};
     }
   }
+}
+
+// One thread per (pair, bin) of a chunk: the largest key over the parts
+// (keys are distinct per lag, so exactly one part holds it) and its value.
+__global__ void __launch_bounds__(256) big_rows_finish_kernel(BigArgs a, long long np) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int nb = (int)a.nb;
+  if (i >= np * nb) return;
+  const long long pl = i / nb;
+  const int b = (int)(i % nb);
+  int d = 0, tt = 0;
+  if (!big_pair(a, a.p0 + pl, d, tt)) return;
+  uint64_t best = 0ull;
+  cf bv = cf{0.f, 0.f};
+  for (int w = 0; w < a.nsplit; w++) {
+    const long long part = (pl * a.nsplit + w) * (long long)nb + b;
+    const uint64_t key = a.pkey[part];
+    if (key > best) {
+      best = key;
+      bv = a.pval[part];
+    }
+  }
+  const long long o = a.list_ooff ? a.list_ooff[a.p0 + pl] : ((long long)d * a.nt + tt) * a.ostride;
+  a.out_idx[o + b] = best ? (int64_t)(~(uint32_t)(best & 0xffffffffull)) : -1;
+  a.out_val[o + b] = best ? bv : cf{0.f, 0.f};
 }
 
 __global__ void roots_kernel(cf *out, int L, int period) {
@@ -219,7 +275,7 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
   cf *scr = (cf *)c->scratch("big_scratch", (size_t)chunk * N * sizeof(cf));
   if (!scr) return -1;
   a.scratch = scr;
-  const size_t lds_cols = 16 * kColStride * sizeof(cf);
+  const size_t lds_cols = (16 * kColStride + N1 + 256) * sizeof(cf);
   const float thr = r.threshold;
   a.seed = thr > 0.f ? (float)((double)thr * (double)thr) : 0.f;
   a.binsize = (uint32_t)r.binsize;
@@ -239,24 +295,42 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
       return -1;
     attr_set = true;
   }
+  // rows workgroups per pair: up to 8 (one row group each at N1 = 128, two at
+  // 256), as long as the per-part tables stay within 64 MiB
+  constexpr int G = 256 / S::SLOTS;
+  const size_t nb_piece = std::min(maxb, nb_total);
+  int nsplit = std::min(8, G);
+  while (nsplit > 1 && (size_t)chunk * nsplit * nb_piece * 16 > (64ull << 20)) nsplit /= 2;
+  a.nsplit = nsplit;
+  a.pkey = nullptr;
+  a.pval = nullptr;
+  if (nsplit > 1) {
+    a.pkey = (uint64_t *)c->scratch("big_part_key", (size_t)chunk * nsplit * nb_piece * sizeof(uint64_t));
+    a.pval = (cf *)c->scratch("big_part_val", (size_t)chunk * nsplit * nb_piece * sizeof(cf));
+    if (!a.pkey || !a.pval) return -1;
+  }
+  Ctx::ProfScope ps(c, r.list ? r.list_prof_name : r.prof_name);
   for (long long p0 = 0; p0 < total; p0 += chunk) {
     const long long np = std::min(chunk, total - p0);
     a.p0 = p0;
-    {
-      Ctx::ProfScope ps(c, r.list ? "refine" : r.prof_name);
-      hipLaunchKernelGGL(big_cols_kernel<N1>, dim3(N1 / 16, (unsigned)np), dim3(256), lds_cols, c->stream, a);
-      if (!hip_ok(hipGetLastError(), "big_cols_kernel")) return -1;
-      for (size_t b0 = 0; b0 < nb_total; b0 += maxb) {
-        const size_t cnt = std::min(maxb, nb_total - b0);
-        a.ws = (uint32_t)(r.ws + b0 * r.binsize);
-        a.we = (uint32_t)std::min(r.we, r.ws + (b0 + cnt) * r.binsize);
-        a.nb = (uint32_t)cnt;
-        a.ostride = (uint32_t)nb_total;
-        a.out_idx = r.out_idx + b0;
-        a.out_val = r.out_val + b0;
-        const size_t lds = (size_t)S::SLOTS * S::XSTRIDE * sizeof(cf) + cnt * (sizeof(uint64_t) + sizeof(cf));
-        hipLaunchKernelGGL(big_rows_kernel<N1>, dim3((unsigned)np), dim3(256), lds, c->stream, a);
-        if (!hip_ok(hipGetLastError(), "big_rows_kernel")) return -1;
+    hipLaunchKernelGGL(big_cols_kernel<N1>, dim3(N1 / 16, (unsigned)np), dim3(256), lds_cols, c->stream, a);
+    if (!hip_ok(hipGetLastError(), "big_cols_kernel")) return -1;
+    for (size_t b0 = 0; b0 < nb_total; b0 += maxb) {
+      const size_t cnt = std::min(maxb, nb_total - b0);
+      a.ws = (uint32_t)(r.ws + b0 * r.binsize);
+      a.we = (uint32_t)std::min(r.we, r.ws + (b0 + cnt) * r.binsize);
+      a.nb = (uint32_t)cnt;
+      a.ostride = (uint32_t)nb_total;
+      a.out_idx = r.out_idx + b0;
+      a.out_val = r.out_val + b0;
+      const size_t lds = (size_t)S::SLOTS * S::XSTRIDE * sizeof(cf) + cnt * (sizeof(uint64_t) + sizeof(cf));
+      hipLaunchKernelGGL(big_rows_kernel<N1>, dim3((unsigned)np, (unsigned)nsplit), dim3(256), lds, c->stream, a);
+      if (!hip_ok(hipGetLastError(), "big_rows_kernel")) return -1;
+      if (nsplit > 1) {
+        const long long cells = np * (long long)cnt;
+        hipLaunchKernelGGL(big_rows_finish_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, c->stream, a,
+                           np);
+        if (!hip_ok(hipGetLastError(), "big_rows_finish_kernel")) return -1;
       }
     }
   }

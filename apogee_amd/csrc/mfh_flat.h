@@ -105,10 +105,10 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
 // the runs, at 16 the upper quarter of theirs, at 8 one run -- and three
 // plain steps finish it: 10 exchanges instead of 48.  One lane per run then
 // raises the bin's key in LDS.
-template <int TPT>
-__device__ __forceinline__ uint64_t pair_run_max(const cf *v, int t, int r, float seed) {
-  const uint64_t a = make_key(v[2 * r], (uint32_t)(t + 2 * r * TPT), seed);
-  const uint64_t b = make_key(v[2 * r + 1], (uint32_t)(t + (2 * r + 1) * TPT), seed);
+template <int STEP>
+__device__ __forceinline__ uint64_t pair_run_max(const cf *v, uint32_t lag0, int r, float seed) {
+  const uint64_t a = make_key(v[2 * r], lag0 + (uint32_t)(2 * r * STEP), seed);
+  const uint64_t b = make_key(v[2 * r + 1], lag0 + (uint32_t)((2 * r + 1) * STEP), seed);
   return a > b ? a : b;
 }
 
@@ -117,17 +117,20 @@ __device__ __forceinline__ uint64_t xchg_max(uint64_t keep, uint64_t send, int m
   return got > keep ? got : keep;
 }
 
-// R = 8 runs of two u each (bins of exactly 2 TPT lags), streamed so that at
-// most four partial maxima are live at once.
-template <int TPT>
-__device__ __forceinline__ void runs8_max_to_bins(const cf *v, int t, int lane, float seed, int bshift,
-                                                  uint64_t *bins) {
+// R = 8 runs of two u each, streamed so that at most four partial maxima
+// are live at once.  Lane lags are lag0 + u*STEP; lane 0's lag0 is wave_lag0,
+// every lane's lag0 lies in [wave_lag0, wave_lag0 + STEP), and bins are 2 STEP
+// wide starting at lag 0, so run rho of every lane lies in bin
+// (wave_lag0 + 2 rho STEP) >> bshift.
+template <int STEP>
+__device__ __forceinline__ void runs8_max_to_bins(const cf *v, uint32_t lag0, uint32_t wave_lag0, int lane,
+                                                  float seed, int bshift, uint64_t *bins) {
   const bool up1 = (lane & 32) != 0, up2 = (lane & 16) != 0, up3 = (lane & 8) != 0;
   uint64_t q[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    const uint64_t lo = pair_run_max<TPT>(v, t, i, seed);
-    const uint64_t hi = pair_run_max<TPT>(v, t, i + 4, seed);
+    const uint64_t lo = pair_run_max<STEP>(v, lag0, i, seed);
+    const uint64_t hi = pair_run_max<STEP>(v, lag0, i + 4, seed);
     q[i] = xchg_max(up1 ? hi : lo, up1 ? lo : hi, 32);
   }
   const uint64_t r0 = xchg_max(up2 ? q[2] : q[0], up2 ? q[0] : q[2], 16);
@@ -138,7 +141,7 @@ __device__ __forceinline__ void runs8_max_to_bins(const cf *v, int t, int lane, 
   w = xchg_max(w, w, 1);
   const int rho = (up1 ? 4 : 0) + (up2 ? 2 : 0) + (up3 ? 1 : 0);
   if ((lane & 7) == 0 && w) {
-    const uint32_t b = ((uint32_t)(t - lane) + (uint32_t)(rho * 2 * TPT)) >> bshift;
+    const uint32_t b = (wave_lag0 + (uint32_t)(rho * 2 * STEP)) >> bshift;
     atomicMax((unsigned long long *)&bins[b], (unsigned long long)w);
   }
 }
@@ -151,7 +154,7 @@ __device__ __forceinline__ bool full_window_bins(const cf *v, int t, int lane, f
                                                  uint64_t *bins) {
   constexpr int LT = ilog2(TPT);
   if (bshift - LT != 1) return false;
-  runs8_max_to_bins<TPT>(v, t, lane, seed, bshift, bins);
+  runs8_max_to_bins<TPT>(v, (uint32_t)t, (uint32_t)(t - lane), lane, seed, bshift, bins);
   return true;
 }
 

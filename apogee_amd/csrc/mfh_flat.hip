@@ -89,7 +89,7 @@ int launch_n(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, si
   int mode = kGeneral;
   if (nb == 1) mode = kOneBin;
   else if (ws % 64 == 0 && (we % 64 == 0) && r.binsize % 64 == 0) mode = kAligned;
-  Ctx::ProfScope ps(c, list ? "refine" : r.prof_name);
+  Ctx::ProfScope ps(c, list ? r.list_prof_name : r.prof_name);
   if (list) {
     if (mode == kOneBin) return launch_mode<N, kOneBin, true>(c, a, grid, lds);
     if (mode == kAligned) return launch_mode<N, kAligned, true>(c, a, grid, lds);

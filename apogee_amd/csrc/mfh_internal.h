@@ -95,7 +95,8 @@ struct FlatRun {
   const int *list_count = nullptr;
   const long long *list_ooff = nullptr;  // output element offset per pair (else d*nt + t rows)
   int list_cap = 0;
-  const char *prof_name = "flat";
+  const char *prof_name = "flat";        // event-timing name of a tiled launch
+  const char *list_prof_name = "refine";  // and of a list-mode launch
 };
 
 bool flat_supported(size_t n);

@@ -2,21 +2,26 @@
 // (data spectrum, template spectrum) pairs whose transform lengths differ.
 //
 // Work queue.  A plan turns requests (one data spectrum, c templates, a
-// length n, binsize, threshold and window each) into items, one per pair,
-// grouped into tiles of equal length: a tile is as many pairs as one
-// 1024-lane workgroup carries side by side (1024 / (n/16) for n <= 16384, one
-// pair for the four-step lengths).  Tiles are ordered longest first, so the
-// last tiles to be taken are the shortest and the tail of the launch stays
-// short.  Every workgroup takes the next tile with one atomic on a device
-// counter, runs it and repeats until the counter passes the tile count, so
-// each workgroup reaches the exit.  One workgroup of 1024 lanes per CU.
+// length n, binsize, threshold and window each) into items, one per pair.
+// Pairs of length n <= 16384 are grouped into tiles of one length, as many
+// pairs as one 1024-lane workgroup carries side by side (1024 / (n/16)).
+// Tiles are ordered longest first, so the last tiles to be taken are the
+// shortest and the tail of the launch stays short.  Every workgroup takes the
+// next tile with one atomic on a device counter, runs it and repeats until
+// the counter passes the tile count, so each workgroup reaches the exit.  One
+// workgroup of 1024 lanes per CU.
 //
-// Arithmetic.  A pair of length n <= 16384 runs the same product,
-// inverse_fft<n> and twiddle tables as the flat kernel; the four-step lengths
-// run the same columns/rows split, roots and tables as the two-kernel path,
-// here inside one workgroup with a per-workgroup scratch.  Selection is the
-// flat kernel's exact key rule, so indices and values equal the per-request
-// flat filter bit for bit.
+// Four-step lengths (32768, 65536) need a round trip through scratch between
+// their column and row transforms; inside one resident workgroup that round
+// trip is exposed latency (measured: time per pair independent of how many
+// workgroups share the device).  Their pairs therefore go, as a device list
+// with per-pair pool and output offsets, to the two-kernel four-step path of
+// the flat filter (columns of 16 lags per workgroup, then rows), queued just
+// before the persistent launch in the same stream and graph.
+//
+// Arithmetic.  Every pair runs the same product, transforms, twiddle tables
+// and exact key rule as the per-request flat filter, so indices and values
+// equal it bit for bit.
 //
 // Reference behaviour: each pair is one `ap_mf_run` / `Context.peaks` call of
 // the flat filter (src/matchfilt.c:349-420, python/matchedfilter/_vkcompute.py:962);
@@ -48,21 +53,15 @@ struct QTile {
 };
 
 constexpr int kQThreads = 1024;
-constexpr int kQMaxLog2 = 16;
+constexpr int kQMaxLog2 = 14;  // longest length of the persistent kernel
 // exchange space of a tile: 1024 lanes x 16 points, padded by 1/16, plus one
 // element per slot where slots share a wave (largest at n = 64: 256 slots)
 template <int N>
 constexpr size_t q_xch_bytes() {
   return (size_t)(kQThreads / FlatShape<N>::TPT) * FlatShape<N>::XSTRIDE * sizeof(cf);
 }
-constexpr size_t kQXch = q_xch_bytes<64>();
-static_assert(q_xch_bytes<128>() <= kQXch && q_xch_bytes<4096>() <= kQXch && q_xch_bytes<16384>() <= kQXch, "");
-static_assert((size_t)64 * kColStride * sizeof(cf) <= kQXch, "four-step columns");
-static_assert(q_xch_bytes<128>() <= kQXch && q_xch_bytes<256>() <= kQXch, "four-step rows");
-// per-bin (key, value) table of a four-step pair, after the exchange space;
-// with the 16-byte tile slot the total is the 160 KiB a workgroup may hold
-constexpr int kQBigBins = (int)((160 * 1024 - 16 - kQXch) / (sizeof(uint64_t) + sizeof(cf)));
-constexpr size_t kQLds = kQXch + (size_t)kQBigBins * (sizeof(uint64_t) + sizeof(cf));
+constexpr size_t kQLds = q_xch_bytes<64>();
+static_assert(q_xch_bytes<128>() <= kQLds && q_xch_bytes<4096>() <= kQLds && q_xch_bytes<16384>() <= kQLds, "");
 
 struct QArgs {
   const cf *pool;
@@ -70,9 +69,7 @@ struct QArgs {
   const QTile *tiles;
   int ntiles;
   int *counter;
-  const cf *tw[kQMaxLog2 + 1];  // flat twiddle tables by log2 length (and 128, 256)
-  const cf *roots[2];           // four-step roots: n = 32768, 65536
-  cf *scratch;                  // [gridDim.x][65536] four-step intermediate
+  const cf *tw[kQMaxLog2 + 1];  // flat twiddle tables by log2 length
   int64_t *out_idx;
   cf *out_val;
 };
@@ -94,7 +91,7 @@ __device__ __forceinline__ void q_select(const cf *v, int t, int tid, const QIte
   if constexpr (TPT >= 64) {
     // whole-transform window in n/8 bins: the transposing butterfly
     if (it.ws == 0 && it.we == 16u * TPT && it.bshift == ilog2(TPT) + 1) {
-      runs8_max_to_bins<TPT>(v, t, lane, it.seed, it.bshift, bkey);
+      runs8_max_to_bins<TPT>(v, (uint32_t)t, (uint32_t)(t - lane), lane, it.seed, it.bshift, bkey);
       return;
     }
   }
@@ -186,107 +183,6 @@ __device__ __forceinline__ void q_flat(const QArgs &a, const QTile &tile, cf *sm
   }
 }
 
-// One four-step pair, n = N1 x 256 (f = f1 + N1 f2, k = k2 + 256 k1), inside
-// the workgroup: columns (64 per round: four groups of 16 adjacent f1, each
-// column a 256-point transform on 16 lanes) go to this workgroup's scratch,
-// then rows (1024 / (N1/16) per round) transform and raise a per-bin
-// (key, value) table in LDS; after each round the holder of a bin's key
-// records its value.
-template <int N1>
-__device__ __forceinline__ void q_big(const QArgs &a, const QTile &tile, cf *smem, int tid) {
-  constexpr int N = N1 * 256;
-  const QItem it = a.items[tile.first];
-  const cf *D = a.pool + it.d_off;
-  const cf *T = a.pool + it.t_off;
-  cf *Y = a.scratch + (long long)blockIdx.x * 65536;
-  const cf *rootH = a.roots[N1 == 256];
-  const cf *rootL = rootH + N1;
-  {
-    const int s = tid & 15;
-    const int t = (tid >> 4) & 15;
-    const int g = tid >> 8;
-    cf *xch = smem + (g * 16 + s) * kColStride;
-    for (int r = 0; r < N1 / 64; r++) {
-      const int f1 = (r * 4 + g) * 16 + s;
-      cf v[16];
-#pragma unroll
-      for (int u = 0; u < 16; u++) {
-        const int f = f1 + N1 * (t + 16 * u);
-        v[u] = cmul_conj(D[f], T[f]);
-      }
-      inverse_fft<256>(v, t, xch, a.tw[8]);
-#pragma unroll
-      for (int u = 0; u < 16; u++) {
-        const int k2 = t + 16 * u;
-        const int m = f1 * k2;
-        const cf w = cmul(rootH[m >> 8], rootL[m & 255]);
-        Y[(long long)k2 * N1 + f1] = cmul(v[u], w);
-      }
-      __syncthreads();  // exchange reusable; after the last round, Y is visible
-    }
-  }
-  constexpr int TPT = N1 / 16;
-  constexpr int SLOTS = kQThreads / TPT;
-  const int slot = tid / TPT;
-  const int t = tid % TPT;
-  cf *xch = smem + slot * FlatShape<N1>::XSTRIDE;
-  uint64_t *bkey = reinterpret_cast<uint64_t *>(reinterpret_cast<unsigned char *>(smem) + kQXch);
-  cf *bval = reinterpret_cast<cf *>(bkey + kQBigBins);
-  const int nb = (int)it.nb;
-  for (int b = tid; b < nb; b += kQThreads) bkey[b] = 0ull;
-  __syncthreads();
-  for (int r = 0; r < 256 / SLOTS; r++) {
-    const int k2 = r * SLOTS + slot;
-    cf v[16];
-#pragma unroll
-    for (int u = 0; u < 16; u++) v[u] = Y[(long long)k2 * N1 + t + TPT * u];
-    inverse_fft<N1>(v, t, xch, a.tw[ilog2(N1)]);
-    uint64_t acc = 0ull;
-    int acc_bin = -1;
-#pragma unroll
-    for (int u = 0; u < 16; u++) {
-      const uint32_t k = (uint32_t)(k2 + 256 * (t + TPT * u));
-      if (k >= it.ws && k < it.we) {
-        const uint64_t key = make_key(v[u], k, it.seed);
-        const int b = (int)q_bin(k, it);
-        if (b != acc_bin) {
-          if (acc) atomicMax((unsigned long long *)&bkey[acc_bin], (unsigned long long)acc);
-          acc = 0ull;
-          acc_bin = b;
-        }
-        acc = key > acc ? key : acc;
-      }
-    }
-    if (acc) atomicMax((unsigned long long *)&bkey[acc_bin], (unsigned long long)acc);
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 16; u++) {
-      const uint32_t k = (uint32_t)(k2 + 256 * (t + TPT * u));
-      if (k >= it.ws && k < it.we) {
-        const uint64_t key = make_key(v[u], k, it.seed);
-        if (key) {
-          const uint32_t b = q_bin(k, it);
-          if (bkey[b] == key) bval[b] = v[u];
-        }
-      }
-    }
-    __syncthreads();  // values settled, exchange reusable
-  }
-  int64_t *oidx = a.out_idx + it.o_off;
-  cf *oval = a.out_val + it.o_off;
-  for (int b = tid; b < nb; b += kQThreads) {
-    const uint64_t key = bkey[b];
-    if (key) {
-      oidx[b] = (int64_t)(~(uint32_t)(key & 0xffffffffull));
-      oval[b] = bval[b];
-    } else {
-      oidx[b] = -1;
-      oval[b] = cf{0.f, 0.f};
-    }
-  }
-  (void)N;
-}
-
 __global__ void __launch_bounds__(kQThreads) queue_kernel(QArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ int tile_slot;
@@ -311,8 +207,6 @@ __global__ void __launch_bounds__(kQThreads) queue_kernel(QArgs a) {
       case 12: q_flat<4096>(a, tile, smem, tid); break;
       case 13: q_flat<8192>(a, tile, smem, tid); break;
       case 14: q_flat<16384>(a, tile, smem, tid); break;
-      case 15: q_big<128>(a, tile, smem, tid); break;
-      case 16: q_big<256>(a, tile, smem, tid); break;
       default: break;
     }
     __syncthreads();  // tile done: LDS and tile_slot reusable
@@ -338,10 +232,21 @@ struct Queue {
   int ntiles = 0;
   long long nout = 0;
   long long npairs = 0;
-  int max_log2 = 0;
+  // four-step pairs, grouped by (length, binsize, threshold, window): each
+  // group is one list-mode launch of the two-kernel path
+  struct BigGroup {
+    int n;
+    long long binsize;
+    float threshold;
+    long long ws, we;
+    int first, count;  // entries in the device list
+  };
+  std::vector<BigGroup> big;
   std::vector<long long> req_out;  // output offset per request
   std::vector<int> req_nb, req_nc;
   int grid_per_cu = 1;
+  int grid_cap = 0;  // if > 0: at most this many persistent workgroups
+  long prof_runs = 0;  // runs recorded while event timing is on
 };
 
 }  // namespace mfh
@@ -414,6 +319,8 @@ extern "C" long long mfh_queue_plan(mfh_queue *queue, int nreq, const int64_t *n
   if (nreq && (!n || !data_off || !tmpl_off || !ncand || !binsize || !threshold || !ws || !we))
     return fail("null request array");
   std::vector<std::vector<QItem>> by_class(kQMaxLog2 + 1);
+  std::vector<Queue::BigGroup> bigs;
+  std::vector<std::vector<std::pair<int2, long long>>> big_pairs;
   q->req_out.assign(nreq, 0);
   q->req_nb.assign(nreq, 0);
   q->req_nc.assign(nreq, 0);
@@ -438,24 +345,39 @@ extern "C" long long mfh_queue_plan(mfh_queue *queue, int nreq, const int64_t *n
     q->req_nb[r] = (int)nb;
     q->req_nc[r] = ncand[r];
     if (out_off) out_off[r] = nout;
-    QItem base{};
-    base.d_off = data_off[r];
-    base.seed = thr > 0.f ? (float)((double)thr * (double)thr) : 0.f;
-    base.binsize = (uint32_t)binsize[r];
-    const bool pow2 = (binsize[r] & (binsize[r] - 1)) == 0;
-    base.bshift = pow2 ? ilog2((int)binsize[r]) : -1;
-    base.inv_bs = 1.0f / (float)binsize[r];
-    // four-step pairs keep at most kQBigBins bins per item: longer windows
-    // are split at bin boundaries into items that share the pair
-    const long long piece = lg >= 15 ? kQBigBins : nb;
-    for (int j = 0; j < ncand[r]; j++) {
-      for (long long b0 = 0; b0 < nb; b0 += piece) {
+    if (lg >= 15) {
+      // four-step: one list entry per pair, grouped by launch parameters
+      if (data_off[r] > INT32_MAX || tmpl_off[r] + (long long)ncand[r] * len > INT32_MAX)
+        return fail("request %d: four-step spectra must start below pool element 2^31", r);
+      size_t g = 0;
+      for (; g < bigs.size(); g++) {
+        const auto &G = bigs[g];
+        if (G.n == (int)len && G.binsize == binsize[r] && !memcmp(&G.threshold, &thr, sizeof thr) &&
+            G.ws == ws[r] && G.we == we[r])
+          break;
+      }
+      if (g == bigs.size()) {
+        bigs.push_back({(int)len, (long long)binsize[r], thr, (long long)ws[r], (long long)we[r], 0, 0});
+        big_pairs.emplace_back();
+      }
+      for (int j = 0; j < ncand[r]; j++)
+        big_pairs[g].push_back({int2{(int)data_off[r], (int)(tmpl_off[r] + (long long)j * len)},
+                                nout + (long long)j * nb});
+    } else {
+      QItem base{};
+      base.d_off = data_off[r];
+      base.seed = thr > 0.f ? (float)((double)thr * (double)thr) : 0.f;
+      base.binsize = (uint32_t)binsize[r];
+      const bool pow2 = (binsize[r] & (binsize[r] - 1)) == 0;
+      base.bshift = pow2 ? ilog2((int)binsize[r]) : -1;
+      base.inv_bs = 1.0f / (float)binsize[r];
+      base.ws = (uint32_t)ws[r];
+      base.we = (uint32_t)we[r];
+      base.nb = (uint32_t)nb;
+      for (int j = 0; j < ncand[r]; j++) {
         QItem it = base;
         it.t_off = tmpl_off[r] + (long long)j * len;
-        it.o_off = nout + (long long)j * nb + b0;
-        it.ws = (uint32_t)(ws[r] + b0 * binsize[r]);
-        it.we = (uint32_t)std::min<long long>(we[r], ws[r] + (b0 + piece) * binsize[r]);
-        it.nb = (uint32_t)std::min<long long>(piece, nb - b0);
+        it.o_off = nout + (long long)j * nb;
         by_class[lg].push_back(it);
       }
     }
@@ -464,12 +386,10 @@ extern "C" long long mfh_queue_plan(mfh_queue *queue, int nreq, const int64_t *n
   }
   std::vector<QItem> items;
   std::vector<QTile> tiles;
-  q->max_log2 = 0;
   for (int lg = kQMaxLog2; lg >= 6; lg--) {  // longest first
     auto &v = by_class[lg];
     if (v.empty()) continue;
-    q->max_log2 = std::max(q->max_log2, lg);
-    const int slots = lg >= 15 ? 1 : kQThreads / ((1 << lg) / 16);
+    const int slots = kQThreads / ((1 << lg) / 16);
     for (size_t i = 0; i < v.size(); i += slots) {
       QTile t{};
       t.log2n = lg;
@@ -479,29 +399,40 @@ extern "C" long long mfh_queue_plan(mfh_queue *queue, int nreq, const int64_t *n
     }
     items.insert(items.end(), v.begin(), v.end());
   }
-  if (items.size() > (size_t)INT32_MAX) return fail("too many pairs in one plan");
-  hipSetDevice(q->c.dev);
-  // tables for every length in the plan
-  for (int lg = 6; lg <= kQMaxLog2; lg++) {
-    if (by_class[lg].empty()) continue;
-    if (lg <= 14) {
-      if (!q->c.twiddles((size_t)1 << lg)) return -1;
-    } else {
-      if (!q->c.twiddles(256) || !q->c.twiddles((size_t)1 << (lg - 8)) || !big_roots(&q->c, 1 << (lg - 8)))
-        return -1;
+  std::vector<int2> blist;
+  std::vector<long long> booff;
+  std::vector<int> bcount;
+  for (size_t g = 0; g < bigs.size(); g++) {
+    bigs[g].first = (int)blist.size();
+    bigs[g].count = (int)big_pairs[g].size();
+    bcount.push_back(bigs[g].count);
+    for (auto &e : big_pairs[g]) {
+      blist.push_back(e.first);
+      booff.push_back(e.second);
     }
   }
+  if (items.size() > (size_t)INT32_MAX || blist.size() > (size_t)INT32_MAX) return fail("too many pairs in one plan");
+  hipSetDevice(q->c.dev);
+  // tables for every length in the plan
+  for (int lg = 6; lg <= kQMaxLog2; lg++)
+    if (!by_class[lg].empty() && !q->c.twiddles((size_t)1 << lg)) return -1;
   QItem *ditems = (QItem *)q->c.scratch("q_items", std::max<size_t>(1, items.size()) * sizeof(QItem));
   QTile *dtiles = (QTile *)q->c.scratch("q_tiles", std::max<size_t>(1, tiles.size()) * sizeof(QTile));
-  if (!ditems || !dtiles) return -1;
-  if ((!items.empty() && !hip_ok(hipMemcpyAsync(ditems, items.data(), items.size() * sizeof(QItem),
-                                                hipMemcpyHostToDevice, q->c.stream),
-                                 "copy items")) ||
-      (!tiles.empty() && !hip_ok(hipMemcpyAsync(dtiles, tiles.data(), tiles.size() * sizeof(QTile),
-                                                hipMemcpyHostToDevice, q->c.stream),
-                                 "copy tiles")) ||
+  int2 *dblist = (int2 *)q->c.scratch("q_big_list", std::max<size_t>(1, blist.size()) * sizeof(int2));
+  long long *dbooff = (long long *)q->c.scratch("q_big_ooff", std::max<size_t>(1, booff.size()) * sizeof(long long));
+  int *dbcount = (int *)q->c.scratch("q_big_count", std::max<size_t>(1, bcount.size()) * sizeof(int));
+  if (!ditems || !dtiles || !dblist || !dbooff || !dbcount) return -1;
+  auto up = [&](void *dst, const void *src, size_t bytes, const char *what) {
+    return bytes == 0 || hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, q->c.stream), what);
+  };
+  if (!up(ditems, items.data(), items.size() * sizeof(QItem), "copy items") ||
+      !up(dtiles, tiles.data(), tiles.size() * sizeof(QTile), "copy tiles") ||
+      !up(dblist, blist.data(), blist.size() * sizeof(int2), "copy four-step list") ||
+      !up(dbooff, booff.data(), booff.size() * sizeof(long long), "copy four-step offsets") ||
+      !up(dbcount, bcount.data(), bcount.size() * sizeof(int), "copy four-step counts") ||
       !hip_ok(hipStreamSynchronize(q->c.stream), "hipStreamSynchronize"))
     return -1;
+  q->big = bigs;
   q->nreq = nreq;
   q->ntiles = (int)tiles.size();
   q->nout = nout;
@@ -517,28 +448,45 @@ static int queue_launch(Queue *q, int64_t *idx, cf *val) {
   Ctx *c = &q->c;
   int *counter = (int *)c->scratch("q_counter", sizeof(int));
   if (!counter) return -1;
+  if (c->prof_on && !c->capturing) q->prof_runs++;
+  // four-step pairs first: list-mode launches of the two-kernel path
+  const int2 *blist = (const int2 *)c->bufs["q_big_list"].p;
+  const long long *booff = (const long long *)c->bufs["q_big_ooff"].p;
+  const int *bcount = (const int *)c->bufs["q_big_count"].p;
+  for (size_t g = 0; g < q->big.size(); g++) {
+    const Queue::BigGroup &G = q->big[g];
+    FlatRun r;
+    r.data = q->pool;  // list entries are element offsets into the pool
+    r.tmpl = q->pool;
+    r.data_stride = 1;
+    r.tmpl_stride = 1;
+    r.nd = 1;
+    r.nt = 1;
+    r.binsize = (size_t)G.binsize;
+    r.threshold = G.threshold;
+    r.ws = (size_t)G.ws;
+    r.we = (size_t)G.we;
+    r.out_idx = idx;
+    r.out_val = (cf *)val;
+    r.list = blist + G.first;
+    r.list_count = bcount + g;
+    r.list_ooff = booff + G.first;
+    r.list_cap = G.count;
+    r.list_prof_name = "queue_big";
+    if (launch_big(c, (size_t)G.n, r) < 0) return -1;
+  }
   QArgs a{};
   a.pool = q->pool;
   a.items = (const QItem *)c->bufs["q_items"].p;
   a.tiles = (const QTile *)c->bufs["q_tiles"].p;
   a.ntiles = q->ntiles;
   a.counter = counter;
-  for (int lg = 6; lg <= 14; lg++) {
+  for (int lg = 6; lg <= kQMaxLog2; lg++) {
     auto it = c->tw_by_n.find((size_t)1 << lg);
     a.tw[lg] = it == c->tw_by_n.end() ? nullptr : it->second;
   }
   int grid = std::max(1, std::min(q->cus * q->grid_per_cu, q->ntiles));
-  if (q->max_log2 >= 15) {
-    a.scratch = (cf *)c->scratch("q_scratch", (size_t)grid * 65536 * sizeof(cf));
-    if (!a.scratch) return -1;
-    for (int k = 0; k < 2; k++) {
-      const int n1 = k ? 256 : 128;
-      char rname[32];
-      snprintf(rname, sizeof rname, "bigroots%d", n1 * 256);
-      auto b = c->bufs.find(rname);
-      a.roots[k] = b == c->bufs.end() ? nullptr : (const cf *)b->second.p;
-    }
-  }
+  if (q->grid_cap > 0) grid = std::min(grid, q->grid_cap);
   a.out_idx = idx;
   a.out_val = val;
   hipLaunchKernelGGL(q_zero_kernel, dim3(1), dim3(64), 0, c->stream, counter);
@@ -556,7 +504,7 @@ static int queue_run_cached(Queue *q, int64_t *idx, cf *val) {
   Ctx *c = &q->c;
   if (!c->use_graphs || c->prof_on) return queue_launch(q, idx, val);
   std::vector<long long> key = {q->plan_gen, (long long)(intptr_t)q->pool, (long long)(intptr_t)idx,
-                                (long long)(intptr_t)val, q->grid_per_cu};
+                                (long long)(intptr_t)val, q->grid_per_cu, q->grid_cap};
   if (c->graph_exec && key == c->graph_key) {
     return hip_ok(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch") ? 0 : -1;
   }
@@ -642,6 +590,11 @@ extern "C" int mfh_queue_set_tuning(mfh_queue *queue, const char *key, long valu
     q->grid_per_cu = (int)value;
     return 0;
   }
+  if (!strcmp(key, "grid_cap")) {
+    if (value < 0) return fail("grid_cap must be >= 0");
+    q->grid_cap = (int)value;
+    return 0;
+  }
   if (!strcmp(key, "graphs")) {
     q->c.use_graphs = value != 0;
     return 0;
@@ -666,6 +619,7 @@ extern "C" int mfh_queue_prof_enable(mfh_queue *queue, int on) {
   q->c.prof_collect();
   q->c.prof_on = on != 0;
   q->c.prof_totals.clear();
+  q->prof_runs = 0;
   return 0;
 }
 
@@ -673,8 +627,12 @@ extern "C" int mfh_queue_prof_read(mfh_queue *queue, double *total_ms, long *cou
   if (!queue) return fail("null queue");
   Queue *q = reinterpret_cast<Queue *>(queue);
   q->c.prof_collect();
-  auto it = q->c.prof_totals.find("queue");
-  if (total_ms) *total_ms = it == q->c.prof_totals.end() ? 0.0 : it->second.first;
-  if (count) *count = it == q->c.prof_totals.end() ? 0 : it->second.second;
+  double ms = 0.0;
+  for (const char *name : {"queue", "queue_big"}) {
+    auto it = q->c.prof_totals.find(name);
+    if (it != q->c.prof_totals.end()) ms += it->second.first;
+  }
+  if (total_ms) *total_ms = ms;
+  if (count) *count = q->prof_runs;
   return 0;
 }

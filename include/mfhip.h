@@ -260,11 +260,13 @@ int mfh_queue_run_async(mfh_queue *queue, int64_t *dev_idx, void *dev_val_c64);
 /* Run into host outputs; counts (may be null) receives the reported bins per
  * pair in plan order.  Returns the number of reported bins or -1. */
 long long mfh_queue_run(mfh_queue *queue, int64_t *idx, void *val_c64, int32_t *counts);
-/* Knobs: "grid_per_cu" (persistent workgroups per CU, default 1), "graphs". */
+/* Knobs: "grid_per_cu" (persistent workgroups per CU, default 1), "grid_cap"
+ * (upper bound on persistent workgroups, 0 = none), "graphs". */
 int mfh_queue_set_tuning(mfh_queue *queue, const char *key, long value);
 int mfh_queue_sync(mfh_queue *queue);
 void *mfh_queue_stream(mfh_queue *queue);
-/* Event timing of the persistent launch (graph replay disabled meanwhile). */
+/* Event timing of the queue's launches (graph replay disabled meanwhile):
+ * total milliseconds over all its kernels and the number of runs timed. */
 int mfh_queue_prof_enable(mfh_queue *queue, int on);
 int mfh_queue_prof_read(mfh_queue *queue, double *total_ms, long *count);
 

@@ -9,8 +9,10 @@
 //     dL/dT_t[f] = sum_{d, bins} conj(g) D_d[f] w^(+f k)
 // Bins that reported nothing (index -1) hold the constant 0 and contribute
 // nothing.  Each output element is one thread's ordered sum (no atomics), so
-// the result is deterministic.  w^m comes from an FP64-generated table of the
-// n roots, indexed by (f k) mod n in exact integer arithmetic.
+// the result is deterministic.  w^m, m = (f k) mod n in exact integer
+// arithmetic, is the product of two FP64-generated roots, w^(256 (m >> 8)) and
+// w^(m & 255), read from a 4 KiB copy of both tables in LDS: the indices are
+// scattered across lanes, which the vector cache serves poorly.
 //
 // The same table gives the peak value itself at a stored lag (a direct sum
 // over f per peak, one wave each), which is how a caller recomputes z at the
@@ -28,13 +30,29 @@ namespace mfh {
 
 namespace {
 
-__global__ void unit_circle_kernel(cf *out, int n) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) {
+// Two-level root table of length n: hi[h] = w^(256 h) for h < max(1, n/256),
+// then lo[l] = w^l for l < 256.
+__global__ void circle_tables_kernel(cf *out, int n) {
+  const int nh = n >= 256 ? n / 256 : 1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nh + 256) {
+    const long long e = i < nh ? 256ll * i : (long long)(i - nh);
     double sn, cs;
-    sincospi(2.0 * (double)k / (double)n, &sn, &cs);
-    out[k] = cf{(float)cs, (float)sn};
+    sincospi(2.0 * (double)(e % n) / (double)n, &sn, &cs);
+    out[i] = cf{(float)cs, (float)sn};
   }
+}
+
+// The block's LDS copy of the two tables; returns (hi, lo) in *lds.
+__device__ __forceinline__ void stage_circle(const cf *tables, int n, cf *lds) {
+  const int nh = n >= 256 ? n / 256 : 1;
+  for (int i = threadIdx.x; i < nh + 256; i += blockDim.x) lds[i] = tables[i];
+  __syncthreads();
+}
+
+__device__ __forceinline__ cf circle(const cf *lds, int n, uint32_t m) {
+  const int nh = n >= 256 ? n / 256 : 1;
+  return cmul(lds[m >> 8], lds[nh + (m & 255u)]);
 }
 
 // dL/dT (TEMPL = true) or dL/dD: one thread per (row, f); the row is
@@ -43,11 +61,13 @@ template <bool TEMPL>
 __global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ other, long long other_stride,
                                                         int nother, const int64_t *__restrict__ idx,
                                                         const cf *__restrict__ gval, int nd, int nt, int nb,
-                                                        const cf *__restrict__ roots, int n, cf *__restrict__ out,
+                                                        const cf *__restrict__ tables, int n, cf *__restrict__ out,
                                                         long long out_stride) {
+  __shared__ cf roots[512];
+  stage_circle(tables, n, roots);
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int row = blockIdx.y;
-  if (f >= n) return;
+  if (f >= n) return;  // after the only barrier
   const uint32_t mask = (uint32_t)n - 1u;
   float ax = 0.f, ay = 0.f;
   for (int j = 0; j < nother; j++) {
@@ -58,7 +78,7 @@ __global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ o
       const long long k = idx[pair * nb + b];
       if (k < 0) continue;
       const cf g = gval[pair * nb + b];
-      const cf w = roots[((uint32_t)f * (uint32_t)k) & mask];
+      const cf w = circle(roots, n, ((uint32_t)f * (uint32_t)k) & mask);
       // templates: conj(g) x w ; data: g x conj(w)
       cf xw;
       if (TEMPL) {
@@ -79,10 +99,12 @@ __global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ o
 __global__ void __launch_bounds__(256) peak_value_kernel(const cf *__restrict__ D, long long dstride,
                                                          const cf *__restrict__ T, long long tstride, int nt,
                                                          int nb, long long npeaks, const int64_t *__restrict__ idx,
-                                                         const cf *__restrict__ roots, int n, cf *__restrict__ out) {
+                                                         const cf *__restrict__ tables, int n, cf *__restrict__ out) {
+  __shared__ cf roots[512];
+  stage_circle(tables, n, roots);
   const long long p = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (p >= npeaks) return;  // uniform per wave
+  if (p >= npeaks) return;  // uniform per wave, after the only barrier
   const long long k = idx[p];
   const long long pair = p / nb;
   const int d = (int)(pair / nt), t = (int)(pair % nt);
@@ -92,7 +114,7 @@ __global__ void __launch_bounds__(256) peak_value_kernel(const cf *__restrict__ 
     const cf *dp = D + (long long)d * dstride;
     const cf *tp = T + (long long)t * tstride;
     for (int f = lane; f < n; f += 64) {
-      const cf pr = cmul(cmul_conj(dp[f], tp[f]), roots[((uint32_t)f * (uint32_t)k) & mask]);
+      const cf pr = cmul(cmul_conj(dp[f], tp[f]), circle(roots, n, ((uint32_t)f * (uint32_t)k) & mask));
       sx += pr.x;
       sy += pr.y;
     }
@@ -112,11 +134,12 @@ static const cf *unit_circle(Ctx *c) {
   snprintf(name, sizeof name, "circle%zu", c->n);
   auto it = c->bufs.find(name);
   if (it != c->bufs.end() && it->second.p) return (const cf *)it->second.p;
-  cf *r = (cf *)c->scratch(name, c->n * sizeof(cf));
+  const int nh = c->n >= 256 ? (int)(c->n / 256) : 1;
+  cf *r = (cf *)c->scratch(name, (size_t)(nh + 256) * sizeof(cf));
   if (!r) return nullptr;
-  hipLaunchKernelGGL(unit_circle_kernel, dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0, c->stream, r,
+  hipLaunchKernelGGL(circle_tables_kernel, dim3((unsigned)((nh + 256 + 255) / 256)), dim3(256), 0, c->stream, r,
                      (int)c->n);
-  if (!hip_ok(hipGetLastError(), "unit_circle_kernel")) return nullptr;
+  if (!hip_ok(hipGetLastError(), "circle_tables_kernel")) return nullptr;
   return r;
 }
 

@@ -56,43 +56,70 @@ __device__ __forceinline__ cf circle(const cf *lds, int n, uint32_t m) {
 }
 
 // dL/dT (TEMPL = true) or dL/dD: one thread per (row, f); the row is
-// blockIdx.y, the other side's rows are summed over.
+// blockIdx.y, the other side's rows are summed over.  The row's (lag,
+// gradient) list is staged through LDS in chunks, so the inner loop reads
+// LDS instead of waiting on one global load per term.
 template <bool TEMPL>
 __global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ other, long long other_stride,
                                                         int nother, const int64_t *__restrict__ idx,
                                                         const cf *__restrict__ gval, int nd, int nt, int nb,
                                                         const cf *__restrict__ tables, int n, cf *__restrict__ out,
                                                         long long out_stride) {
+  constexpr int CH = 1024;  // (lag, gradient) entries per LDS chunk
   __shared__ cf roots[512];
+  __shared__ uint32_t sk[CH];
+  __shared__ cf sg[CH];
   stage_circle(tables, n, roots);
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int row = blockIdx.y;
-  if (f >= n) return;  // after the only barrier
+  const bool valid = f < n;
   const uint32_t mask = (uint32_t)n - 1u;
   float ax = 0.f, ay = 0.f;
-  for (int j = 0; j < nother; j++) {
-    // pair (d, t): templates sum over data rows j, data sums over templates j
-    const long long pair = TEMPL ? (long long)j * nt + row : (long long)row * nt + j;
-    const cf x = other[(long long)j * other_stride + f];
-    for (int b = 0; b < nb; b++) {
+  const long long total = (long long)nother * nb;
+  for (long long e0 = 0; e0 < total; e0 += CH) {
+    const int cnt = (int)(total - e0 < CH ? total - e0 : CH);
+    __syncthreads();  // the previous chunk is consumed
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const long long e = e0 + i;
+      const int j = (int)(e / nb), b = (int)(e % nb);
+      const long long pair = TEMPL ? (long long)j * nt + row : (long long)row * nt + j;
       const long long k = idx[pair * nb + b];
-      if (k < 0) continue;
-      const cf g = gval[pair * nb + b];
-      const cf w = circle(roots, n, ((uint32_t)f * (uint32_t)k) & mask);
-      // templates: conj(g) x w ; data: g x conj(w)
-      cf xw;
-      if (TEMPL) {
-        xw = cmul(x, w);
-        ax += g.x * xw.x + g.y * xw.y;
-        ay += g.x * xw.y - g.y * xw.x;
-      } else {
-        xw = cmul_conj(x, w);
-        ax += g.x * xw.x - g.y * xw.y;
-        ay += g.x * xw.y + g.y * xw.x;
+      sk[i] = k < 0 ? 0xffffffffu : (uint32_t)k;
+      sg[i] = gval[pair * nb + b];
+    }
+    __syncthreads();
+    if (valid) {
+      int j = (int)(e0 / nb), b = (int)(e0 % nb), jx = -1;
+      cf x = cf{0.f, 0.f};
+      for (int i = 0; i < cnt; i++) {
+        const uint32_t k = sk[i];
+        if (k != 0xffffffffu) {
+          if (j != jx) {
+            x = other[(long long)j * other_stride + f];
+            jx = j;
+          }
+          const cf g = sg[i];
+          const cf w = circle(roots, n, ((uint32_t)f * k) & mask);
+          // templates: conj(g) x w ; data: g x conj(w)
+          cf xw;
+          if (TEMPL) {
+            xw = cmul(x, w);
+            ax += g.x * xw.x + g.y * xw.y;
+            ay += g.x * xw.y - g.y * xw.x;
+          } else {
+            xw = cmul_conj(x, w);
+            ax += g.x * xw.x - g.y * xw.y;
+            ay += g.x * xw.y + g.y * xw.x;
+          }
+        }
+        if (++b == nb) {
+          b = 0;
+          j++;
+        }
       }
     }
   }
-  out[(long long)row * out_stride + f] = cf{ax, ay};
+  if (valid) out[(long long)row * out_stride + f] = cf{ax, ay};
 }
 
 // z at stored lags: one wave per (pair, bin); FP32 products, FP64 sums.

@@ -73,6 +73,30 @@ constexpr int kColStride = FlatShape<256>::XCH + 1;
 
 __device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 4); }
 
+// The 16 points t + u*TPT of a row whose address is uniform across the wave,
+// through a buffer descriptor: one vector offset (8 t) for the thread and the
+// per-u offset 8 u TPT as a scalar, so the strided loads need no vector
+// address arithmetic.  The descriptor covers exactly the row's n elements.
+template <int N>
+__device__ __forceinline__ void load_row_uniform(cf *dst, const cf *row, int t) {
+  constexpr int TPT = N / 16;
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(row), (short)0, N * (int)sizeof(cf), 0x00020000);
+#pragma unroll
+  for (int u = 0; u < 16; u++) {
+    const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, t * (int)sizeof(cf), u * TPT * (int)sizeof(cf), 0);
+    dst[u] = __builtin_bit_cast(cf, x);
+  }
+}
+
+// A wave-uniform row pointer, made provably uniform (scalar registers).
+__device__ __forceinline__ const cf *uniform_ptr(const cf *p) {
+  const unsigned long long u = (unsigned long long)p;
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+  return (const cf *)(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ uint32_t bin_of(uint32_t k, const FlatArgs &a) {
   const uint32_t x = k - a.ws;
   if (a.bshift >= 0) return x >> a.bshift;
@@ -363,12 +387,23 @@ flat_kernel(FlatArgs a) {
       tt = pr.y;
       const cf *dp = a.data + (long long)d * a.data_stride;
       const cf *tp = a.tmpl + (long long)tt * a.tmpl_stride;
+      if constexpr (TPT >= 64) {
+        // one slot per wave: the pair, and so both rows, are wave-uniform
+        cf dv[16], tv[16];
+        load_row_uniform<N>(dv, uniform_ptr(dp), t);
+        load_row_uniform<N>(tv, uniform_ptr(tp), t);
 #pragma unroll
-      for (int u = 0; u < 16; u++) v[u] = cmul_conj(dp[t + u * TPT], tp[t + u * TPT]);
+        for (int u = 0; u < 16; u++) v[u] = cmul_conj(dv[u], tv[u]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = cmul_conj(dp[t + u * TPT], tp[t + u * TPT]);
+      }
     } else {
-      const cf *dp = a.data + (long long)d * a.data_stride;
+      const cf *dp = a.data + (long long)d * a.data_stride;  // uniform: d is the tile's row
+      cf dv[16];
+      load_row_uniform<N>(dv, dp, t);
 #pragma unroll
-      for (int u = 0; u < 16; u++) v[u] = cmul_conj(dp[t + u * TPT], T[u]);
+      for (int u = 0; u < 16; u++) v[u] = cmul_conj(dv[u], T[u]);
     }
     inverse_fft<N>(v, t, xch, a.twiddle);
 
@@ -377,14 +412,27 @@ flat_kernel(FlatArgs a) {
       float bm = a.seed;
       int bu = -1;
       cf bz = cf{0.f, 0.f};
+      if (a.ws == 0 && a.we == (uint32_t)N) {
+        // the whole transform: no per-lag window test (uniform branch)
 #pragma unroll
-      for (int u = 0; u < 16; u++) {
-        const uint32_t k = (uint32_t)(t + u * TPT);
-        const float m2 = __builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y);
-        if (k >= a.ws && k < a.we && m2 > bm) {  // strict: the lowest lag keeps a tie
-          bm = m2;
-          bu = u;
-          bz = v[u];
+        for (int u = 0; u < 16; u++) {
+          const float m2 = __builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y);
+          if (m2 > bm) {  // strict: the lowest lag keeps a tie
+            bm = m2;
+            bu = u;
+            bz = v[u];
+          }
+        }
+      } else {
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const uint32_t k = (uint32_t)(t + u * TPT);
+          const float m2 = __builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y);
+          if (k >= a.ws && k < a.we && m2 > bm) {  // strict: the lowest lag keeps a tie
+            bm = m2;
+            bu = u;
+            bz = v[u];
+          }
         }
       }
       const uint32_t kb = (uint32_t)(t + bu * TPT);

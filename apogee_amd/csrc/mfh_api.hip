@@ -308,6 +308,11 @@ extern "C" int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value) {
     c->use_graphs = value != 0;
     return 0;
   }
+  if (!strcmp(key, "coarse_rows")) {
+    if (value < 1 || value > 64) return fail("coarse_rows must be in [1, 64]");
+    c->coarse_rows = (int)value;
+    return 0;
+  }
   if (!strcmp(key, "refine_grid")) {
     if (value < 1) return fail("refine_grid must be >= 1");
     c->refine_grid_cap = (int)value;

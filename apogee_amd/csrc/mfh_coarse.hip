@@ -20,6 +20,8 @@
 // screen keeps a superset of the FP32 screen's pairs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <vector>
 
@@ -152,15 +154,25 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
 
   const int d0 = blockIdx.y * rows_per_wave;
   const int d1 = min(nd, d0 + rows_per_wave);
+  // the next row's samples are loaded while this row is transformed
+  cf xn[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) xn[i] = data[(long long)d0 * data_stride + 16 * (8 * q + i) + r];
   for (int d = d0; d < d1; d++) {
-    const cf *D = data + (long long)d * data_stride;
+    cf x[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[i] = xn[i];
+    if (d + 1 < d1) {
+      const cf *Dn = data + (long long)(d + 1) * data_stride;
+#pragma unroll
+      for (int i = 0; i < 8; i++) xn[i] = Dn[16 * (8 * q + i) + r];
+    }
     cf p[8];
     float dmax = 0.f;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      const cf x = D[16 * (8 * q + i) + r];
-      dmax = fmaxf(dmax, fmaxf(fabsf(x.x), fabsf(x.y)));
-      p[i] = cmul_conj(x, ct[i]);
+      dmax = fmaxf(dmax, fmaxf(fabsf(x[i].x), fabsf(x[i].y)));
+      p[i] = cmul_conj(x[i], ct[i]);
     }
     dmax = wave_max_f32(dmax);
     const float sd = dmax > 0.f ? 0.5f / dmax : 0.f;  // |P| <= 1 after scaling
@@ -237,7 +249,7 @@ int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, const cf *
   if (c->coarse_band != 512) return fail("the MFMA coarse screen is built for band 512");
   const cf *roots = coarse_roots(c);
   if (!roots) return -1;
-  const int rows = 4;
+  const int rows = std::max(1, c->coarse_rows);
   dim3 grid((unsigned)((nt + 3) / 4), (unsigned)((nd + rows - 1) / rows));
   if (grid.y > 65535) return fail("too many data rows per launch (%d)", nd);
   Ctx::ProfScope ps(c, "coarse");

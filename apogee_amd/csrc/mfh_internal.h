@@ -32,6 +32,7 @@ struct Ctx {
   // MFMA; the MFMA statistic is widened by (1 + coarse_delta) at the gate
   int coarse_prec = 0;
   float coarse_delta = 0.f;
+  int coarse_rows = 4;  // data rows per wave of the MFMA coarse screen
   bool coarse_roots_ready = false;
   // four-step lengths (32768, 65536): roots built, scratch budget per chunk
   bool big_roots_ready[2] = {false, false};

@@ -196,7 +196,9 @@ long mfh_hier_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, i
                          float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
                          int64_t *n_refined);
 
-/* Tuning knobs: key "dpw" (data rows per workgroup; 0 = automatic).
+/* Tuning knobs: key "dpw" (data rows per workgroup; 0 = automatic),
+ * "refine_grid" (workgroups of a list-driven refine), "coarse_rows" (data
+ * rows per wave of the matrix-core coarse screen), "graphs" (0/1).
  * Replaces: MF_MFTILE / MF_PBMAX environment knobs (src/matchfilt.c:93-107). */
 int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value);
 

@@ -205,6 +205,92 @@ def hier_leg(args):
     return out
 
 
+def large_n_leg(args, n=65536, nd=4, nt=128, binsize=1024, threshold=5.5):
+    """n = 65536, 4 data x 128 templates, 64 bins of 1024 lags, threshold 5.5
+    on sigma-normalised |z| (templates: FFT of white Gaussian series divided by
+    sigma_t); data: unit Gaussian noise with 5% of pairs carrying their
+    template at SNR 8.  One step = the two-kernel four-step flat filter."""
+    import torch
+    import apogee_amd as mf
+    rng = np.random.default_rng(SEED + 10)
+    t_time = rng.standard_normal((nt, n))
+    T = np.fft.fft(t_time)
+    T = (T / np.sqrt(n * np.sum(np.abs(T) ** 2, axis=1))[:, None]).astype(np.complex64)
+    x = rng.standard_normal((nd, n))
+    pairs = rng.random((nd, nt)) < 0.05
+    for d, t in zip(*np.nonzero(pairs)):
+        x[d] += np.roll(t_time[t], int(rng.integers(0, n))) * 8.0 / np.linalg.norm(t_time[t])
+    D = np.fft.fft(x).astype(np.complex64)
+    f = mf.MatchedFilter(n, nd, nt, device="gpu:0")
+    f.set_data(D)
+    f.set_templates(T)
+    ctx = f._gpu
+    nb = -(-n // binsize)
+    idx = torch.empty((nd, nt, nb), dtype=torch.int64, device="cuda")
+    val = torch.empty((nd, nt, nb), dtype=torch.complex64, device="cuda")
+
+    def step():
+        ctx.peaks_async(0, nd, 0, nt, binsize, threshold, 0, n, idx.data_ptr(), val.data_ptr())
+
+    for _ in range(5):
+        step()
+    ctx.sync()
+    steps = max(10, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.prof_enable(True)
+    for _ in range(steps):
+        step()
+    ms, cnt = ctx.prof_read("flat")
+    ctx.prof_enable(False)
+    kern_s = ms * 1e-3 / max(1, cnt)
+    got = idx.cpu().numpy()
+    found = float(((got >= 0).any(axis=2) & pairs).sum() / max(1, pairs.sum()))
+    flops = nd * nt * flops_per_pair(n)
+    return {"workload": "n=65536, 4 data x 128 templates, 64 bins of 1024 lags, threshold 5.5 "
+                        "(sigma-normalised), 5% SNR-8 injections; two-kernel four-step",
+            "pairs_per_s": nd * nt / dt, "ms_per_step": dt * 1e3, "kernel_us": kern_s * 1e6,
+            "roofline": {"bound": "valu", "achieved": flops / kern_s / 1e12, "peak": PEAK_FP32_TFLOPS,
+                         "unit": "TFLOP/s", "frac": flops / kern_s / 1e12 / PEAK_FP32_TFLOPS},
+            "injected_detection_rate": found}
+
+
+def series_leg(args):
+    """run_series on a real float32 time series held on the device, blocks at
+    50% overlap (starts b n/2), windows [n/4, 3n/4): n = 4096 with 16 blocks x
+    1024 templates (one bin) and n = 65536 with 4 blocks x 128 templates (bins
+    of 1024).  Wall time of the public call: block transforms, filter and the
+    result copy."""
+    import torch
+    import apogee_amd as mf
+    out = {}
+    for n, nblk, nt, bs in ((4096, 16, 1024, 4096), (65536, 4, 128, 1024)):
+        rng = np.random.default_rng(SEED + n)
+        T = np.fft.fft(rng.standard_normal((nt, n)))
+        T = (T / np.sqrt(n * np.sum(np.abs(T) ** 2, axis=1))[:, None]).astype(np.complex64)
+        series = torch.from_numpy(rng.standard_normal(nblk * n // 2 + n // 2).astype(np.float32)).to("cuda")
+        starts = (np.arange(nblk) * (n // 2)).astype(np.int64)
+        ws = np.full(nblk, n // 4, np.int64)
+        we = np.full(nblk, 3 * n // 4, np.int64)
+        f = mf.MatchedFilter(n, 1, nt, device="gpu:0")
+        f.set_templates(T)
+        ser = series.to(torch.complex64)
+        for _ in range(3):
+            f.run_series(ser, starts, ws, we, binsize=bs, threshold=5.5)
+        steps = max(5, args.steps // 20)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f.run_series(ser, starts, ws, we, binsize=bs, threshold=5.5)
+        dt = (time.perf_counter() - t0) / steps
+        out["n%d" % n] = {"blocks": nblk, "templates": nt, "binsize": bs,
+                          "block_template_pairs_per_s": nblk * nt / dt, "ms_per_call": dt * 1e3}
+    return out
+
+
 def uneven_leg(args, nreq=4000):
     """Uneven batched refine: one persistent launch over 4000 requests with n
     in {1024, 4096, 16384, 65536} (weights .4/.3/.2/.1), binsize n/8 and
@@ -316,6 +402,7 @@ def main():
     ap.add_argument("--no-hier", action="store_true", help="skip the hierarchical leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven refine leg")
     ap.add_argument("--no-fit", action="store_true", help="skip the peak-fit (autograd) leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the n=65536 and run_series legs")
     ap.add_argument("--fd-batches", type=int, default=64,
                     help="independent data batches for the measured false dismissal")
     ap.add_argument("--copy-probe", action="store_true",
@@ -397,6 +484,9 @@ def main():
     }
     if not args.no_hier:
         out["hierarchical"] = hier_leg(args)
+    if not args.no_large:
+        out["large_n"] = large_n_leg(args)
+        out["run_series"] = series_leg(args)
     if not args.no_uneven:
         out["uneven_refine"] = uneven_leg(args)
     if not args.no_fit:

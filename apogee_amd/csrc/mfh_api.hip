@@ -308,6 +308,11 @@ extern "C" int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value) {
     c->use_graphs = value != 0;
     return 0;
   }
+  if (!strcmp(key, "big_chunk_mb")) {
+    if (value < 1 || value > 65536) return fail("big_chunk_mb must be in [1, 65536]");
+    c->big_chunk_bytes = (long long)value << 20;
+    return 0;
+  }
   if (!strcmp(key, "coarse_rows")) {
     if (value < 1 || value > 64) return fail("coarse_rows must be in [1, 64]");
     c->coarse_rows = (int)value;

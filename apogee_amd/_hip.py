@@ -65,6 +65,7 @@ def _declare(lib):
     lib.mfh_hier_run_series.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, _c_size, ctypes.c_float,
                                         ctypes.c_float, _vp, _vp, _vp, _vp]
+    lib.mfh_series_forward.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int]
     lib.mfh_hier_set_coarse_precision.argtypes = [_vp, ctypes.c_int, ctypes.c_float]
     lib.mfh_hier_coarse_max.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         _c_size, _c_size, ctypes.c_int, _vp]
@@ -350,6 +351,19 @@ class Context:
         check(lib().mfh_peak_values_async(self.handle, vp(data_ptr), self.n, int(nd), vp(tmpl_ptr), self.n,
                                           int(nt), int(nb), vp(idx_ptr), vp(out_ptr)),
               "mfh_peak_values_async")
+
+    def series_forward(self, series, starts, row0=0):
+        """Block spectra of a series into data rows [row0, row0 + len(starts))."""
+        dev = _device_pointer(series)
+        if dev is not None:
+            import torch
+            torch.cuda.current_stream(series.device).synchronize()
+            sp, ns, on_dev = ctypes.c_void_p(dev), int(series.numel()), 1
+        else:
+            sp, ns, on_dev = _ptr(series), int(series.size), 0
+        st = np.ascontiguousarray(starts, dtype=np.uintp)
+        check(lib().mfh_series_forward(self.handle, sp, ns, on_dev, _ptr(st), int(st.size), int(row0)),
+              "mfh_series_forward")
 
     # ------------------------------------------------------------- tuning
     def set_tuning(self, key, value):

@@ -298,3 +298,34 @@ extern "C" long mfh_hier_run_series(mfh_ctx *ctx, const void *series_c64, size_t
   return run_series(reinterpret_cast<Ctx *>(ctx), true, series_c64, nseries, series_on_device, start, win_start,
                     win_end, nblocks, t0, nt, binsize, threshold, coarse_thr, idx, val_c64, counts, n_refined);
 }
+
+// Block spectra into data rows: the forward half of run_series on its own.
+extern "C" int mfh_series_forward(mfh_ctx *ctx, const void *series_c64, size_t nseries, int series_on_device,
+                                  const size_t *start, int nblocks, int row0) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (nblocks < 1) return fail("series_forward needs at least one block");
+  if (!series_c64 || !start) return fail("null argument");
+  if (row0 < 0 || row0 + nblocks > c->nd) return fail("data rows [%d, %d) out of bounds", row0, row0 + nblocks);
+  hipSetDevice(c->dev);
+  long long *dstart = (long long *)c->scratch("series_starts", (size_t)nblocks * sizeof(long long));
+  if (!dstart) return -1;
+  const cf *dser;
+  if (series_on_device) {
+    dser = (const cf *)series_c64;
+  } else {
+    cf *buf = (cf *)c->scratch("series_in", (nseries ? nseries : 1) * sizeof(cf));
+    if (!buf) return -1;
+    if (nseries && !hip_ok(hipMemcpyAsync(buf, series_c64, nseries * sizeof(cf), hipMemcpyHostToDevice, c->stream),
+                           "copy series"))
+      return -1;
+    dser = buf;
+  }
+  std::vector<long long> st(nblocks);
+  for (int b = 0; b < nblocks; b++) st[b] = start[b] < nseries ? (long long)start[b] : (long long)nseries;
+  if (!hip_ok(hipMemcpyAsync(dstart, st.data(), nblocks * sizeof(long long), hipMemcpyHostToDevice, c->stream),
+              "copy starts"))
+    return -1;
+  if (mfh::launch_series_forward(c, dser, nseries, dstart, nblocks, c->data + (size_t)row0 * c->n) < 0) return -1;
+  return hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize") ? 0 : -1;
+}

@@ -185,6 +185,15 @@ long mfh_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, int se
                     int t0, int nt, size_t binsize, float threshold,
                     int64_t *idx, void *val_c64, int32_t *counts);
 
+/* The block spectra of run_series on their own: spectrum b = FFT of
+ * series[start[b] : start[b] + n] (zero-padded past the end) with the 1/n
+ * applied first, written into data rows [row0, row0 + nblocks) of the
+ * context, where mfh_flat_peaks / mfh_hier_peaks then read them.
+ * Replaces: Context.forward (python/matchedfilter/_vkcompute.py:855-903,
+ * src/gpu/series_forward.slang:9-33), which fills the GPU data buffer. */
+int mfh_series_forward(mfh_ctx *ctx, const void *series_c64, size_t nseries, int series_on_device,
+                       const size_t *start, int nblocks, int row0);
+
 /* The hierarchical twin: the block spectra go through the coarse screen and
  * refine of mfh_hier_peaks (band set by mfh_hier_set_band); *n_refined
  * receives the number of refined (block, template) pairs.

@@ -155,3 +155,30 @@ def test_device_resident_series(gpu_device):
     dev = f.run_series(torch.from_numpy(series).to("cuda"), starts, ws, we, binsize=256)
     np.testing.assert_array_equal(host["index"], dev["index"])
     np.testing.assert_array_equal(host["value"], dev["value"])
+
+
+@pytest.mark.parametrize("n", [1024, 65536])
+def test_series_forward_fills_data_rows(n, gpu_device):
+    """mfh_series_forward writes the block spectra run_series uses: filtering
+    those rows equals run_series, and the rows equal FFT(block)/n."""
+    from apogee_amd import _hip
+    rng = np.random.default_rng(7 + n)
+    nt, nblk = 3, 4
+    series = rng.standard_normal(nblk * n // 2 + n // 3).astype(np.complex64)
+    T = (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64)
+    starts = (np.arange(nblk) * (n // 2)).astype(np.uintp)
+    ctx = _hip.Context(gpu_device, n, nblk + 1, nt)
+    ctx.set_rows("template", 0, T)
+    ctx.series_forward(series, starts, row0=1)
+    idx, val, _ = ctx.peaks(1, nblk, 0, nt, n // 8, 0.0, 0, n)
+    f = mf.MatchedFilter(n, 1, nt, device="gpu")
+    f.set_templates(T)
+    got = f.run_series(series, starts, np.zeros(nblk, np.uintp), np.full(nblk, n, np.uintp),
+                       binsize=n // 8, threshold=0.0)
+    np.testing.assert_array_equal(idx, got["index"])
+    np.testing.assert_array_equal(val, got["value"])
+    spec = O.series_spectra(series, starts, n)
+    check_against_oracle(series, starts, np.zeros(nblk, np.uintp), np.full(nblk, n, np.uintp), T, n // 8, 0.0, got)
+    assert spec.shape == (nblk, n)
+    with pytest.raises(RuntimeError):
+        ctx.series_forward(series, starts, row0=2)        # rows 2..5 exceed ndata = 5

@@ -102,13 +102,16 @@ template <bool BF16>
 __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long long data_stride, const cf *ctmpl,
                                                           int nd, int nt, int rows_per_wave, int cs, int ce,
                                                           const cf *w32, const cf *w512, const cf *w16,
-                                                          float *cmax2) {
+                                                          float *cmax2, int *zero) {
   using F = Frag<BF16>;
   using v8 = typename F::v8;
   using v4 = typename F::v4;
   constexpr int M = 512;
   const int lane = threadIdx.x & 63;
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // the refine-list counter of the gate that follows starts at zero; no
+  // kernel appends to it before this one has finished
+  if (zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *zero = 0;
   if (t >= nt) return;  // whole waves exit; no barriers in this kernel
   const int q = lane >> 4, r = lane & 15;
 
@@ -245,7 +248,7 @@ static const cf *coarse_roots(Ctx *c) {
 }
 
 int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, const cf *ctmpl, int nd, int nt, int cs, int ce,
-                       bool bf16, float *cmax2) {
+                       bool bf16, float *cmax2, int *zero) {
   if (c->coarse_band != 512) return fail("the MFMA coarse screen is built for band 512");
   const cf *roots = coarse_roots(c);
   if (!roots) return -1;
@@ -255,10 +258,10 @@ int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, const cf *
   Ctx::ProfScope ps(c, "coarse");
   if (bf16)
     hipLaunchKernelGGL(coarse_mfma_kernel<true>, grid, dim3(256), 0, c->stream, data, data_stride, ctmpl, nd, nt, rows,
-                       cs, ce, roots, roots + 32, roots + 544, cmax2);
+                       cs, ce, roots, roots + 32, roots + 544, cmax2, zero);
   else
     hipLaunchKernelGGL(coarse_mfma_kernel<false>, grid, dim3(256), 0, c->stream, data, data_stride, ctmpl, nd, nt,
-                       rows, cs, ce, roots, roots + 32, roots + 544, cmax2);
+                       rows, cs, ce, roots, roots + 32, roots + 544, cmax2, zero);
   return hip_ok(hipGetLastError(), "coarse_mfma_kernel") ? 0 : -1;
 }
 

@@ -47,6 +47,7 @@ struct FlatArgs {
   int bshift;          // log2(binsize) when a power of two, else -1
   float inv_bs;        // 1 / binsize
   float seed;          // threshold^2 (FP32), or 0
+  int *zero;           // if set, workgroup 0 stores 0 there (a counter for the next kernel)
 };
 
 template <int N>
@@ -364,6 +365,7 @@ flat_kernel(FlatArgs a) {
 #pragma unroll
     for (int u = 0; u < 16; u++) T[u] = valid ? tp[t0 + u * TPT] : cf{0.f, 0.f};
   }
+  if (a.zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *a.zero = 0;
   // both bin buffers start empty
   for (int i = threadIdx.x; i < 2 * S::SLOTS * nb; i += S::WG) bins_all[i] = 0ull;
   __syncthreads();

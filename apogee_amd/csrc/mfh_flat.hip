@@ -72,6 +72,7 @@ int launch_n(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, si
   a.list = r.list;
   a.list_count = r.list_count;
   a.list_ooff = r.list_ooff;
+  a.zero = bin0 == 0 ? r.zero : nullptr;
   const bool list = r.list != nullptr;
   int dpw = c->tune_dpw;
   if (dpw <= 0) dpw = std::min(r.nd, 4);

@@ -59,10 +59,6 @@ __global__ void __launch_bounds__(256) coarse_template_kernel(const cf *tmpl, lo
     ct[(long long)t * band + k] = cf{(float)((double)T[k].x * scale), (float)((double)T[k].y * scale)};
 }
 
-__global__ void zero_count_kernel(int *count) {
-  if (threadIdx.x == 0) *count = 0;
-}
-
 // Gate one pair on its coarse peak: fire iff a peak was reported (|z|^2 above
 // thr^2) and sqrt(|z|^2) >= thr, exactly the CPU engine's two tests.  Fired
 // pairs are appended to the refine list; the rest get -1 / 0 in every bin.
@@ -174,7 +170,7 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
     cmax2 = (float *)c->scratch("coarse_max", pairs * sizeof(float));
     if (!cmax2) return -1;
     if (launch_coarse_mfma(c, drows, (long long)n, c->coarse_tmpl + (size_t)t0 * m, nd, nt, (int)cs, (int)ce,
-                           c->coarse_prec == 2, cmax2) < 0)
+                           c->coarse_prec == 2, cmax2, count) < 0)
       return -1;
   }
   FlatRun cr;
@@ -191,9 +187,9 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   cr.out_idx = cidx;
   cr.out_val = cval;
   cr.prof_name = "coarse";
+  cr.zero = count;  // the coarse launch clears the survivor counter
   if (!mfma && launch_flat(c, m, cr) < 0) return -1;
 
-  hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(64), 0, c->stream, count);
   {
     Ctx::ProfScope ps(c, "gate");
     if (mfma)

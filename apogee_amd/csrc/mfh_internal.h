@@ -98,6 +98,7 @@ struct FlatRun {
   int list_cap = 0;
   const char *prof_name = "flat";        // event-timing name of a tiled launch
   const char *list_prof_name = "refine";  // and of a list-mode launch
+  int *zero = nullptr;                    // if set, the launch stores 0 there
 };
 
 bool flat_supported(size_t n);
@@ -114,9 +115,10 @@ int launch_big(Ctx *c, size_t n, const FlatRun &r);
 const cf *big_roots(Ctx *c, int n1);
 // Forward spectra of run_series blocks into out [nblocks][n] (device).
 int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out);
-// MFMA coarse statistic (max |z_c|^2 over [cs, ce)) per pair, band 512.
+// MFMA coarse statistic (max |z_c|^2 over [cs, ce)) per pair, band 512;
+// `zero` (may be null) is set to 0 by the launch.
 int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, const cf *ctmpl, int nd, int nt, int cs, int ce,
-                       bool bf16, float *cmax2);
+                       bool bf16, float *cmax2, int *zero = nullptr);
 // Hierarchical run over data rows [d0, d0+nd) of the context, or over
 // `data_rows` (stride n) when not null.
 int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,

@@ -155,6 +155,7 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
     gn[i] = F::cvt4(-w.y);
   }
 
+  const bool full = cs == 0 && ce == M;  // whole coarse window: no per-lag test
   const int d0 = blockIdx.y * rows_per_wave;
   const int d1 = min(nd, d0 + rows_per_wave);
   // the next row's samples are loaded while this row is transformed
@@ -206,11 +207,16 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
       zi = mfma16x16x16(gr, bi, zi);
       zi = mfma16x16x16(gi, br, zi);
       // z at j = c + 32 e with c = r + 16 h, e = 4q + i
+      if (full) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int j = r + 16 * h + 32 * (4 * q + i);
-        const float m2 = __builtin_fmaf(zr[i], zr[i], zi[i] * zi[i]);
-        if (j >= cs && j < ce) best = fmaxf(best, m2);
+        for (int i = 0; i < 4; i++) best = fmaxf(best, __builtin_fmaf(zr[i], zr[i], zi[i] * zi[i]));
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int j = r + 16 * h + 32 * (4 * q + i);
+          const float m2 = __builtin_fmaf(zr[i], zr[i], zi[i] * zi[i]);
+          if (j >= cs && j < ce) best = fmaxf(best, m2);
+        }
       }
     }
     best = wave_max_f32(best);

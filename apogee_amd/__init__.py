@@ -31,6 +31,9 @@ PEAK_DTYPE = np.dtype([("index", "<i8"), ("value", "<c8")])
 #: answered by ``_hip.supported`` (others raise UnsupportedSize).
 _GPU_SIZES = frozenset((64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384,
                         32768, 65536))
+#: Lengths this build adds beyond the reference's GPU set (its CPU engine's
+#: larger power-of-two lengths), carried by the same four-step split.
+_LARGE_SIZES = frozenset((131072, 262144, 524288, 1048576))
 
 __all__ = ["MatchedFilter", "HierarchicalFilter", "RefineQueue", "PEAK_DTYPE", "devices",
            "Device", "UnsupportedSize", "__version__"]
@@ -134,9 +137,9 @@ class MatchedFilter:
         self.ntemplates = int(ntemplates)
         if self.ndata < 1 or self.ntemplates < 1:
             raise ValueError("ndata and ntemplates must be >= 1")
-        if self.n not in _GPU_SIZES:
+        if self.n not in _GPU_SIZES | _LARGE_SIZES:
             raise ValueError(
-                "device='gpu' supports n in %s; got %d" % (sorted(_GPU_SIZES), self.n))
+                "device='gpu' supports n in %s; got %d" % (sorted(_GPU_SIZES | _LARGE_SIZES), self.n))
         self.device = _parse_device(device)
         self._init_state()
         self._gpu = self._context_factory(self.device.index, self.n, self.ndata, self.ntemplates)

@@ -295,11 +295,13 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
       return -1;
     attr_set = true;
   }
-  // rows workgroups per pair: up to 8 (one row group each at N1 = 128, two at
-  // 256), as long as the per-part tables stay within 64 MiB
+  // rows workgroups per pair: enough for about 1024 workgroups per launch (at
+  // least 8, at most one per row group), as long as the per-part tables stay
+  // within 64 MiB
   constexpr int G = 256 / S::SLOTS;
   const size_t nb_piece = std::min(maxb, nb_total);
-  int nsplit = std::min(8, G);
+  int nsplit = (int)std::min<long long>(G, std::max<long long>(8, (1024 + chunk - 1) / chunk));
+  nsplit = 1 << ilog2(nsplit);  // a power of two: parts split the row groups evenly
   while (nsplit > 1 && (size_t)chunk * nsplit * nb_piece * 16 > (64ull << 20)) nsplit /= 2;
   a.nsplit = nsplit;
   a.pkey = nullptr;
@@ -345,19 +347,25 @@ const cf *big_roots(Ctx *c, int n1) {
   snprintf(rname, sizeof rname, "bigroots%d", n);
   cf *roots = (cf *)c->scratch(rname, (n1 + 256) * sizeof(cf));
   if (!roots) return nullptr;
-  if (!c->big_roots_ready[n1 == 256]) {
+  if (!c->big_roots_ready[ilog2(n1)]) {
     hipLaunchKernelGGL(roots_kernel, dim3((n1 + 255) / 256), dim3(256), 0, c->stream, roots, n1, n1);
     hipLaunchKernelGGL(roots_kernel, dim3(1), dim3(256), 0, c->stream, roots + n1, 256, n);
     if (!hip_ok(hipGetLastError(), "roots_kernel")) return nullptr;
-    c->big_roots_ready[n1 == 256] = true;
+    c->big_roots_ready[ilog2(n1)] = true;
   }
   return roots;
 }
 
 int launch_big(Ctx *c, size_t n, const FlatRun &r) {
-  if (n == 32768) return launch_big_n1<128>(c, r);
-  if (n == 65536) return launch_big_n1<256>(c, r);
-  return fail("transform length %zu is not a four-step length", n);
+  switch (n) {
+    case 32768: return launch_big_n1<128>(c, r);
+    case 65536: return launch_big_n1<256>(c, r);
+    case 131072: return launch_big_n1<512>(c, r);
+    case 262144: return launch_big_n1<1024>(c, r);
+    case 524288: return launch_big_n1<2048>(c, r);
+    case 1048576: return launch_big_n1<4096>(c, r);
+    default: return fail("transform length %zu is not a four-step length", n);
+  }
 }
 
 }  // namespace mfh

@@ -178,7 +178,7 @@ int flat_fill_twiddles(Ctx *c, size_t n, cf *tw) {
 bool flat_supported(size_t n) {
   switch (n) {
     case 64: case 128: case 256: case 512: case 1024: case 2048: case 4096: case 8192: case 16384:
-    case 32768: case 65536:
+    case 32768: case 65536: case 131072: case 262144: case 524288: case 1048576:
       return true;
     default:
       return false;
@@ -203,7 +203,11 @@ int launch_flat(Ctx *c, size_t n, const FlatRun &r) {
     case 8192: return launch_len<8192>(c, q);
     case 16384: return launch_len<16384>(c, q);
     case 32768:
-    case 65536: return launch_big(c, n, q);
+    case 65536:
+    case 131072:
+    case 262144:
+    case 524288:
+    case 1048576: return launch_big(c, n, q);
     default: return fail("transform length %zu is not supported", n);
   }
 }

@@ -187,6 +187,7 @@ extern "C" int mfh_peak_grad_async(mfh_ctx *ctx, const void *dev_data_c64, long 
   if (!dev_data_c64 || !dev_tmpl_c64 || !dev_idx || !dev_gval_c64) return fail("null input pointer");
   if (data_stride < (long long)c->n || tmpl_stride < (long long)c->n) return fail("row stride below n");
   if (nd > 65535 || nt > 65535) return fail("at most 65535 data and template rows per call");
+  if (c->n > 65536) return fail("peak gradients are built for n <= 65536");
   hipSetDevice(c->dev);
   const cf *roots = mfh::unit_circle(c);
   if (!roots) return -1;
@@ -215,6 +216,7 @@ extern "C" int mfh_peak_values_async(mfh_ctx *ctx, const void *dev_data_c64, lon
   if (nd < 1 || nt < 1 || nb < 1) return fail("empty peak array (%d x %d x %zu)", nd, nt, nb);
   if (!dev_data_c64 || !dev_tmpl_c64 || !dev_idx || !dev_out_c64) return fail("null pointer");
   if (data_stride < (long long)c->n || tmpl_stride < (long long)c->n) return fail("row stride below n");
+  if (c->n > 65536) return fail("peak values are built for n <= 65536");
   hipSetDevice(c->dev);
   const cf *roots = mfh::unit_circle(c);
   if (!roots) return -1;

@@ -35,7 +35,7 @@ struct Ctx {
   int coarse_rows = 4;  // data rows per wave of the MFMA coarse screen
   bool coarse_roots_ready = false;
   // four-step lengths (32768, 65536): roots built, scratch budget per chunk
-  bool big_roots_ready[2] = {false, false};
+  bool big_roots_ready[16] = {};  // by log2 of the column length
   long long big_chunk_bytes = 128ll << 20;  // measured best on gfx950 (256: +7%, 64: +16%)
   int refine_grid_cap = 2048;  // workgroups of a list-driven (refine) launch
   // Per-pass twiddle tables of the flat kernel for length n (FP64-generated

@@ -327,7 +327,7 @@ extern "C" long long mfh_queue_plan(mfh_queue *queue, int nreq, const int64_t *n
   long long nout = 0, npairs = 0;
   for (int r = 0; r < nreq; r++) {
     const long long len = n[r];
-    if (len < 64 || len > 65536 || (len & (len - 1)))
+    if (len < 64 || len > 1048576 || (len & (len - 1)))
       return fail("request %d: transform length %lld is not supported", r, len);
     if (ncand[r] < 0) return fail("request %d: negative candidate count", r);
     if (binsize[r] < 1) return fail("request %d: binsize must be >= 1", r);

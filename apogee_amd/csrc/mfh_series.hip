@@ -169,6 +169,10 @@ int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long l
     case 16384: return launch_fwd<16384>(c, series, nseries, starts, nblocks, out);
     case 32768: return launch_fwd_big<128>(c, series, nseries, starts, nblocks, out);
     case 65536: return launch_fwd_big<256>(c, series, nseries, starts, nblocks, out);
+    case 131072: return launch_fwd_big<512>(c, series, nseries, starts, nblocks, out);
+    case 262144: return launch_fwd_big<1024>(c, series, nseries, starts, nblocks, out);
+    case 524288: return launch_fwd_big<2048>(c, series, nseries, starts, nblocks, out);
+    case 1048576: return launch_fwd_big<4096>(c, series, nseries, starts, nblocks, out);
     default: return fail("transform length %zu is not supported", c->n);
   }
 }

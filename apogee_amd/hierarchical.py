@@ -32,7 +32,8 @@ _COARSE_PRECISIONS = {"fp32": (0, 0.0), "fp16": (1, 2e-3), "bf16": (2, 1.6e-2)}
 #: Coarse band per transform length on gfx950 (power of two, 64 <= band < n).
 #: At n = 4096 the 512-point band is what the benchmark configuration pins.
 _GFX950_BAND = {1024: 128, 2048: 256, 4096: 512, 8192: 1024, 16384: 2048,
-                32768: 4096, 65536: 8192}
+                32768: 4096, 65536: 8192, 131072: 16384, 262144: 32768, 524288: 65536,
+                1048576: 131072}
 
 
 class HierarchicalFilter(MatchedFilter):

@@ -20,7 +20,7 @@ import numpy as np
 from . import _hip
 from ._errors import UnsupportedSize
 
-_LENGTHS = frozenset(1 << k for k in range(6, 17))
+_LENGTHS = frozenset(1 << k for k in range(6, 21))
 
 
 def _peak_dtype():
@@ -243,9 +243,9 @@ class RefineQueue:
     @staticmethod
     def _check_length(n):
         if n not in _LENGTHS:
-            if 64 <= n <= 65536 and n & (n - 1) == 0:
+            if 64 <= n <= 1048576 and n & (n - 1) == 0:
                 raise UnsupportedSize("transform length %d is not supported" % n)
-            raise ValueError("transform length must be a power of two in [64, 65536], got %d" % n)
+            raise ValueError("transform length must be a power of two in [64, 1048576], got %d" % n)
 
     @staticmethod
     def _bins(n, binsize, window):

@@ -48,7 +48,7 @@ def test_no_cpu_fallback_when_the_library_is_missing(monkeypatch, tmp_path):
 
 def test_supported_lengths_query():
     from apogee_amd import _hip
-    for n in (64, 1024, 4096, 16384, 32768, 65536):
+    for n in (64, 1024, 4096, 16384, 32768, 65536, 131072, 1 << 20):
         assert _hip.supported(n)
-    for n in (0, 1000, 1 << 20):
+    for n in (0, 32, 1000, 3 << 16, 1 << 21):
         assert not _hip.supported(n)

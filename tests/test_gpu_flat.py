@@ -399,3 +399,25 @@ def test_four_step_known_lags_and_subranges(gpu_device):
     sub = f.run(binsize=n, data=(0, 2), templates=(2, 3))
     np.testing.assert_array_equal(sub["index"], full["index"][0:2, 2:5])
     np.testing.assert_array_equal(sub["value"], full["value"][0:2, 2:5])
+
+
+@pytest.mark.parametrize("n", [131072, 262144, 524288, 1048576])
+def test_lengths_beyond_the_reference_gpu_set(n, gpu_device):
+    """131072 ... 1048576 through the same four-step split, against the FP64
+    oracle, with a planted lag per pair and n/8 bins and 1000-lag bins."""
+    rng = np.random.default_rng(n)
+    nd, nt = 2, 3
+    T = (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64)
+    D = (rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))).astype(np.complex64)
+    f_ = np.arange(n)
+    for d in range(nd):
+        lag = int(rng.integers(n))
+        D[d] += (0.05 * T[d % nt] * np.exp(2j * np.pi * f_ * lag / n)).astype(np.complex64)
+    f = mf.MatchedFilter(n, nd, nt, device="gpu")
+    f.set_data(D)
+    f.set_templates(T)
+    for binsize, window in ((n // 8, None), (1000, (777, n - 5))):
+        kw = {} if window is None else {"window": window}
+        got = f.run(binsize=binsize, threshold=0.0, **kw)
+        assert_peaks_match(D, T, got["index"], got["value"], binsize, 0.0, window,
+                           context="n=%d binsize=%d" % (n, binsize))

@@ -91,7 +91,7 @@ def test_awkward_combinations(binsize, window, gpu_device):
     assert_one_sided(a, b, "binsize=%d window=%s" % (binsize, window))
 
 
-@pytest.mark.parametrize("n", [1024, 2048, 4096, 8192, 16384, 32768, 65536])
+@pytest.mark.parametrize("n", [1024, 2048, 4096, 8192, 16384, 32768, 65536, 131072])
 def test_every_transform_length(n, gpu_device):
     a, b, _ = run_both(n, 2, 4, n // 8, 5.5, None)
     assert_one_sided(a, b, "n=%d" % n)

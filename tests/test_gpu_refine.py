@@ -88,6 +88,7 @@ def test_ragged_bins_windows_thresholds_and_pieces(gpu_device):
         (16384, 2, 64, 0.0, (100, 16300)), (32768, 1, 1024, 0.0, None), (32768, 2, 5, 2.0, (7, 9000)),
         (65536, 1, 16, 0.0, None),          # 4096 bins: window pieces of the four-step
         (65536, 2, 1000, 4.0, (123, 65000)), (65536, 1, 65536, 0.0, (0, 65536)),
+        (262144, 2, 32768, 0.0, None), (1048576, 1, 4096, 0.0, (100, 1048000)),
         (1024, 0, 64, 0.0, None),           # no candidates
         (4096, 1, 64, 1000.0, None),        # nothing crosses (|z| ~ 64 on this noise)
     ]

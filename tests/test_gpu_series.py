@@ -157,7 +157,7 @@ def test_device_resident_series(gpu_device):
     np.testing.assert_array_equal(host["value"], dev["value"])
 
 
-@pytest.mark.parametrize("n", [1024, 65536])
+@pytest.mark.parametrize("n", [1024, 65536, 262144])
 def test_series_forward_fills_data_rows(n, gpu_device):
     """mfh_series_forward writes the block spectra run_series uses: filtering
     those rows equals run_series, and the rows equal FFT(block)/n."""

@@ -80,7 +80,7 @@ def test_validation(fake):
     with pytest.raises(ValueError):
         q.add(cnoise(rng, (2, 64)), cnoise(rng, (1, 64)))
     with pytest.raises(ValueError):
-        q.add(cnoise(rng, 131072), cnoise(rng, (1, 131072)))
+        q.add(cnoise(rng, 1 << 21), cnoise(rng, (1, 1 << 21)))
     pool = cnoise(rng, 1024)
     with pytest.raises(ValueError):
         q.plan(pool, n=[256], data_offset=[900], template_offset=[0], candidates=[1])

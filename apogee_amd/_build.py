@@ -12,6 +12,12 @@ into two kernels (the flat filter and the refine queue) could round
 differently in the last bit; with this setting every kernel that runs the
 transform produces identical bits, and the flat kernel's time is unchanged
 within noise (measured on gfx950).
+
+Matrix-core accumulators live in ordinary VGPRs
+(``-mllvm -amdgpu-mfma-vgpr-form``): the coarse screen reads every
+accumulator back into VALU arithmetic, and with the AGPR form each read is
+one more VALU instruction (36 of ~200 per pair before the switch).  Only the
+MFMA kernels are affected.
 """
 import os
 import shutil
@@ -38,6 +44,7 @@ def command(out=LIB):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     return [hipcc(), "--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=on",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-mllvm", "-amdgpu-mfma-vgpr-form",
             "-I", os.path.join(ROOT, "include"), "-I", CSRC,
             "-o", out] + srcs
 

@@ -7,6 +7,9 @@ Tunables (each a key of the library's tuning interface):
   reads and fewer, longer workgroups; fewer rows: more workgroups in flight);
 * ``hier`` -- ``refine_grid``: workgroups of the list-driven refine launch,
   which drains however many pairs the coarse gate passed;
+* ``coarse`` -- ``coarse_rpw`` x ``coarse_tpw``: data rows and templates
+  each wave of the matrix-core coarse screen takes (more of either: the
+  per-wave set-up is shared by more pairs; fewer: more waves in flight);
 * ``queue`` -- ``grid_per_cu``: persistent workgroups per CU of the
   mixed-length refine queue.
 
@@ -30,6 +33,7 @@ TABLE_PATH = os.path.join(_HERE, "tuning_gfx950.json")
 CANDIDATES = {
     "flat": {"dpw": [1, 2, 4, 8, 16]},
     "hier": {"refine_grid": [256, 512, 1024, 2048, 4096]},
+    "coarse": {"coarse_rpw": [1, 2], "coarse_tpw": [2, 4, 8, 16]},
     "queue": {"grid_per_cu": [1, 2]},
 }
 
@@ -39,6 +43,7 @@ FLAT_SHAPES = [
     (16384, 16, 256, 16384), (65536, 4, 128, 1024),
 ]
 HIER_SHAPES = [(4096, 16, 1024, 4096, 512)]
+COARSE_SHAPES = [(4096, 16, 1024, 512), (4096, 64, 256, 512), (4096, 4, 4096, 512)]
 QUEUE_WORKLOADS = [("uneven-4000", 4000)]
 
 _table = None
@@ -158,6 +163,29 @@ def _tune_hier(device, n, nd, nt, binsize, band, reps, rng):
             "best": {"refine_grid": best}, "ms": {str(k): round(v, 5) for k, v in ms.items()}}
 
 
+def _tune_coarse(device, n, nd, nt, band, reps, rng):
+    import itertools
+    import numpy as np
+    from . import _hip
+    ctx = _hip.Context(device, n, nd, nt)
+    ctx.set_rows("data", 0, (rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))).astype(np.complex64))
+    ctx.set_rows("template", 0, (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64))
+    ctx.hier_set_band(band, -1.0)
+    ms = {}
+    c = CANDIDATES["coarse"]
+    for rpw, tpw in itertools.product(c["coarse_rpw"], c["coarse_tpw"]):
+        ctx.set_tuning("coarse_rpw", rpw)
+        ctx.set_tuning("coarse_tpw", tpw)
+        run = lambda: ctx.hier_coarse_max(0, nd, 0, nt, 0, n, 1)   # the fp16 screen
+        run()
+        ms[(rpw, tpw)] = _median_ms(run, reps, ctx, "coarse")
+    best = min(ms, key=ms.get)
+    ctx.destroy()
+    return {"op": "coarse", "n": n, "ndata": nd, "ntemplates": nt, "band": band,
+            "best": {"coarse_rpw": best[0], "coarse_tpw": best[1]},
+            "ms": {"%dx%d" % k: round(v, 5) for k, v in ms.items()}}
+
+
 def uneven_workload(nreq, seed, device="cuda"):
     """The uneven-refine plan: n in {1024, 4096, 16384, 65536} with weights
     .4/.3/.2/.1, binsize n/8, candidates ~ geometric(1/3); spectra are seeded
@@ -219,6 +247,10 @@ def main(argv=None):
         entries.append(e)
     for n, nd, nt, bs, band in HIER_SHAPES:
         e = _tune_hier(args.device, n, nd, nt, bs, band, args.reps, rng)
+        print(json.dumps(e), flush=True)
+        entries.append(e)
+    for n, nd, nt, band in COARSE_SHAPES:
+        e = _tune_coarse(args.device, n, nd, nt, band, args.reps, rng)
         print(json.dumps(e), flush=True)
         entries.append(e)
     for wname, nreq in QUEUE_WORKLOADS:

@@ -41,6 +41,7 @@ Ctx::~Ctx() {
   if (dev >= 0) hipSetDevice(dev);
   if (stream) hipStreamSynchronize(stream);
   if (graph_exec) hipGraphExecDestroy(graph_exec);
+  drop_hier_graphs();
   for (auto &kv : bufs) hipFree(kv.second.p);
   for (auto &e : prof_events) {
     hipEventDestroy(e.a);
@@ -73,8 +74,9 @@ void *Ctx::scratch(const char *name, size_t bytes) {
     if (graph_exec) {
       hipGraphExecDestroy(graph_exec);
       graph_exec = nullptr;
-      graph_key.clear();
     }
+    drop_hier_graphs();
+    graph_key.clear();
   }
   if (!hip_ok(hipMalloc(&b.p, bytes ? bytes : 1), "hipMalloc")) return nullptr;
   b.bytes = bytes;
@@ -313,9 +315,14 @@ extern "C" int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value) {
     c->big_chunk_bytes = (long long)value << 20;
     return 0;
   }
-  if (!strcmp(key, "coarse_rows")) {
-    if (value < 1 || value > 64) return fail("coarse_rows must be in [1, 64]");
-    c->coarse_rows = (int)value;
+  if (!strcmp(key, "coarse_rpw")) {
+    if (value != 1 && value != 2) return fail("coarse_rpw must be 1 or 2");
+    c->coarse_rpw = (int)value;
+    return 0;
+  }
+  if (!strcmp(key, "coarse_tpw")) {
+    if (value < 1 || value > 64) return fail("coarse_tpw must be in [1, 64]");
+    c->coarse_tpw = (int)value;
     return 0;
   }
   if (!strcmp(key, "refine_grid")) {

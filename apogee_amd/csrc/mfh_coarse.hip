@@ -13,9 +13,9 @@
 // IS the B-operand layout of the 16x16x16 instruction, so step 3 takes its
 // input straight from registers.
 //
-// Inputs are scaled to unit maximum per template and per data row before the
-// conversion to fp16/bf16 (FP32 accumulation throughout); the result is
-// rescaled.  The statistic's relative error is bounded by a measured factor
+// Inputs are scaled by exact powers of two to below unit maximum per
+// template and per data row before the conversion to fp16/bf16 (FP32
+// accumulation throughout); the result is rescaled exactly.  The statistic's relative error is bounded by a measured factor
 // (mfh_hier_set_coarse_precision) by which the gate is widened, so the
 // screen keeps a superset of the FP32 screen's pairs.
 #include <hip/hip_runtime.h>
@@ -89,104 +89,139 @@ struct Frag<true> {
   }
 };
 
-__device__ __forceinline__ float wave_max_f32(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-  return v;
+// Max over the wave of nonnegative floats, given and returned as their bit
+// patterns (which order like the values), every lane receiving it: four DPP
+// steps leave each row of 16 lanes holding its row maximum (xor 1 and xor 2
+// by quad permutes, then the mirrored half row and the mirrored row), and
+// the four row maxima are read into scalars.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_umax(uint32_t v) {
+  return max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false));
 }
 
-// One wave per template, walking `rows_per_wave` data rows; four waves per
-// workgroup.  Writes the coarse maximum |z_c|^2 over lags [cs, ce) per pair,
-// in units of the unscaled inputs.
-template <bool BF16>
-__global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long long data_stride, const cf *ctmpl,
-                                                          int nd, int nt, int rows_per_wave, int cs, int ce,
-                                                          const cf *w32, const cf *w512, const cf *w16,
-                                                          float *cmax2, int *zero) {
-  using F = Frag<BF16>;
-  using v8 = typename F::v8;
-  using v4 = typename F::v4;
-  constexpr int M = 512;
-  const int lane = threadIdx.x & 63;
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
-  // the refine-list counter of the gate that follows starts at zero; no
-  // kernel appends to it before this one has finished
-  if (zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *zero = 0;
-  if (t >= nt) return;  // whole waves exit; no barriers in this kernel
-  const int q = lane >> 4, r = lane & 15;
+__device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
+  v = dpp_umax<0xB1>(v);   // quad_perm(1, 0, 3, 2)
+  v = dpp_umax<0x4E>(v);   // quad_perm(2, 3, 0, 1)
+  v = dpp_umax<0x141>(v);  // row_half_mirror
+  v = dpp_umax<0x140>(v);  // row_mirror
+  const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+  const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+  const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+  const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+  return max(max(a, b), max(c, d));
+}
 
-  // A-fragment positions of step 1: row b = r, k = a = 8q + i -> f = 16 a + b
-  cf ct[8];
-  float smax = 0.f;
+__device__ __forceinline__ float wave_max_nonneg(float v) { return __uint_as_float(wave_umax(__float_as_uint(v))); }
+
+// Exponent s with m * 2^-s < 1 and m * 2^-s >= 1/2 for a finite m > 0 (the
+// biased exponent of m minus 126); m = 0 gives -126.
+__device__ __forceinline__ int unit_exponent(float m) { return (int)(__float_as_uint(m) >> 23) - 126; }
+
+constexpr int kBand = 512;
+
+// Coarse templates in the step-1 A-fragment order, scaled by an exact power
+// of two below unit maximum: lane l = 16 q + r holds the eight samples
+// f = 128 q + 16 i + r, i < 8, contiguously at ctm[(t * 64 + l) * 8 + i],
+// equal to ct[t][f] * 2^-s_t with s_t = sexp[t] = unit_exponent(max over the
+// template of |Re|, |Im|).  One wave per template; built when the coarse
+// templates are.
+__global__ void __launch_bounds__(64) coarse_mfma_prep_kernel(const cf *ct, cf *ctm, int *sexp) {
+  const int t = blockIdx.x, lane = threadIdx.x, q = lane >> 4, r = lane & 15;
+  cf v[8];
+  float m = 0.f;
 #pragma unroll
   for (int i = 0; i < 8; i++) {
-    ct[i] = ctmpl[(long long)t * M + 16 * (8 * q + i) + r];
-    smax = fmaxf(smax, fmaxf(fabsf(ct[i].x), fabsf(ct[i].y)));
+    v[i] = ct[(long long)t * kBand + 128 * q + 16 * i + r];
+    m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
   }
-  smax = wave_max_f32(smax);
-  const float st = smax > 0.f ? 1.0f / smax : 0.f;
+  const int s = unit_exponent(wave_max_nonneg(m));
+  cf *out = ctm + ((long long)t * 64 + lane) * 8;
 #pragma unroll
-  for (int i = 0; i < 8; i++) ct[i] = cf{ct[i].x * st, ct[i].y * st};
+  for (int i = 0; i < 8; i++) out[i] = cf{ldexpf(v[i].x, -s), ldexpf(v[i].y, -s)};
+  if (lane == 0) sexp[t] = s;
+}
 
-  // step-1 B operand: F32[a][c], a = 8q + i, c = r + 16 h (two column tiles h)
-  v8 fr[2], fi[2], fn[2];
+// The per-lane constants of the factored 512-point DFT.
+template <bool BF16>
+struct CoarseDft {
+  using F = Frag<BF16>;
+  typename F::v8 fr[2], fi[2], fn[2];  // step-1 B operand F32[a][c], a = 8q + i, c = r + 16 h
+  cf tw[2][4];                          // w512^(b c) at the accumulator positions b = 4q + i
+  typename F::v4 gr, gi, gn;            // step-3 A operand F16[e][b], e = r, b = 4q + i
+
+  // Per-lane table of all the constants above (kLaneWords 32-bit words per
+  // lane, lane-major), built once per context by coarse_consts_kernel.
+  static constexpr int kLaneWords = 48;
+  __device__ __forceinline__ void load_table(const uint32_t *tab, int lane) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(tab + lane * kLaneWords);
+    uint4 w[12];
 #pragma unroll
-  for (int h = 0; h < 2; h++)
+    for (int i = 0; i < 12; i++) w[i] = p[i];
+    __builtin_memcpy(&fr[0], &w[0], 16);
+    __builtin_memcpy(&fr[1], &w[1], 16);
+    __builtin_memcpy(&fi[0], &w[2], 16);
+    __builtin_memcpy(&fi[1], &w[3], 16);
+    __builtin_memcpy(&fn[0], &w[4], 16);
+    __builtin_memcpy(&fn[1], &w[5], 16);
+    __builtin_memcpy(&tw[0][0], &w[6], 64);
+    __builtin_memcpy(&gr, &w[10], 8);
+    __builtin_memcpy(&gi, reinterpret_cast<const char *>(&w[10]) + 8, 8);
+    __builtin_memcpy(&gn, &w[11], 8);
+  }
+  __device__ __forceinline__ void store_table(uint32_t *tab, int lane) const {
+    uint4 w[12] = {};
+    __builtin_memcpy(&w[0], &fr[0], 16);
+    __builtin_memcpy(&w[1], &fr[1], 16);
+    __builtin_memcpy(&w[2], &fi[0], 16);
+    __builtin_memcpy(&w[3], &fi[1], 16);
+    __builtin_memcpy(&w[4], &fn[0], 16);
+    __builtin_memcpy(&w[5], &fn[1], 16);
+    __builtin_memcpy(&w[6], &tw[0][0], 64);
+    __builtin_memcpy(&w[10], &gr, 8);
+    __builtin_memcpy(reinterpret_cast<char *>(&w[10]) + 8, &gi, 8);
+    __builtin_memcpy(&w[11], &gn, 8);
+    uint4 *p = reinterpret_cast<uint4 *>(tab + lane * kLaneWords);
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const cf w = w32[((8 * q + i) * (r + 16 * h)) & 31];
-      fr[h][i] = F::cvt8(w.x);
-      fi[h][i] = F::cvt8(w.y);
-      fn[h][i] = F::cvt8(-w.y);
-    }
-  // twiddles w512^(b c) at the accumulator positions b = 4q + i, c = r + 16 h
-  cf tw[2][4];
-#pragma unroll
-  for (int h = 0; h < 2; h++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) tw[h][i] = w512[((4 * q + i) * (r + 16 * h)) & 511];
-  // step-3 A operand: F16[e][b], e = r, b = 4q + i
-  v4 gr, gi, gn;
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const cf w = w16[(r * (4 * q + i)) & 15];
-    gr[i] = F::cvt4(w.x);
-    gi[i] = F::cvt4(w.y);
-    gn[i] = F::cvt4(-w.y);
+    for (int i = 0; i < 12; i++) p[i] = w[i];
   }
 
-  const bool full = cs == 0 && ce == M;  // whole coarse window: no per-lag test
-  const int d0 = blockIdx.y * rows_per_wave;
-  const int d1 = min(nd, d0 + rows_per_wave);
-  // the next row's samples are loaded while this row is transformed
-  cf xn[8];
+  __device__ __forceinline__ void load(const cf *w32, const cf *w512, const cf *w16, int q, int r) {
 #pragma unroll
-  for (int i = 0; i < 8; i++) xn[i] = data[(long long)d0 * data_stride + 16 * (8 * q + i) + r];
-  for (int d = d0; d < d1; d++) {
-    cf x[8];
+    for (int h = 0; h < 2; h++)
 #pragma unroll
-    for (int i = 0; i < 8; i++) x[i] = xn[i];
-    if (d + 1 < d1) {
-      const cf *Dn = data + (long long)(d + 1) * data_stride;
+      for (int i = 0; i < 8; i++) {
+        const cf w = w32[((8 * q + i) * (r + 16 * h)) & 31];
+        fr[h][i] = F::cvt8(w.x);
+        fi[h][i] = F::cvt8(w.y);
+        fn[h][i] = F::cvt8(-w.y);
+      }
 #pragma unroll
-      for (int i = 0; i < 8; i++) xn[i] = Dn[16 * (8 * q + i) + r];
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) tw[h][i] = w512[((4 * q + i) * (r + 16 * h)) & 511];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const cf w = w16[(r * (4 * q + i)) & 15];
+      gr[i] = F::cvt4(w.x);
+      gi[i] = F::cvt4(w.y);
+      gn[i] = F::cvt4(-w.y);
     }
-    cf p[8];
-    float dmax = 0.f;
+  }
+
+  // This lane's maximum of |z[j]|^2 over its 8 lags j = r + 16 h + 32 (4q + i)
+  // that lie in [cs, ce) (all of them when FULL), for the scaled spectra x
+  // (data) and ct (template) at the A-fragment positions.  |z|^2 >= 0, so
+  // the maximum is taken on the bit patterns (no NaN canonicalisation).
+  template <bool FULL>
+  __device__ __forceinline__ uint32_t lane_max(const cf *x, const cf *ct, int cs, int ce, int q, int r) const {
+    typename F::v8 xr, xi;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      dmax = fmaxf(dmax, fmaxf(fabsf(x[i].x), fabsf(x[i].y)));
-      p[i] = cmul_conj(x[i], ct[i]);
+      const cf p = cmul_conj(x[i], ct[i]);
+      xr[i] = F::cvt8(p.x);
+      xi[i] = F::cvt8(p.y);
     }
-    dmax = wave_max_f32(dmax);
-    const float sd = dmax > 0.f ? 0.5f / dmax : 0.f;  // |P| <= 1 after scaling
-    v8 xr, xi;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      xr[i] = F::cvt8(p[i].x * sd);
-      xi[i] = F::cvt8(p[i].y * sd);
-    }
-    float best = 0.f;
+    uint32_t best = 0u;
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       float4v yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
@@ -194,7 +229,7 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
       yr = mfma16x16x32(xi, fn[h], yr);
       yi = mfma16x16x32(xr, fi[h], yi);
       yi = mfma16x16x32(xi, fr[h], yi);
-      v4 br, bi;
+      typename F::v4 br, bi;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const cf y = cmul(cf{yr[i], yi[i]}, tw[h][i]);
@@ -206,23 +241,131 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
       zr = mfma16x16x16(gn, bi, zr);
       zi = mfma16x16x16(gr, bi, zi);
       zi = mfma16x16x16(gi, br, zi);
-      // z at j = c + 32 e with c = r + 16 h, e = 4q + i
-      if (full) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) best = fmaxf(best, __builtin_fmaf(zr[i], zr[i], zi[i] * zi[i]));
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
+      for (int i = 0; i < 4; i++) {
+        const uint32_t m2 = __float_as_uint(__builtin_fmaf(zr[i], zr[i], zi[i] * zi[i]));
+        if (FULL) {
+          best = max(best, m2);
+        } else {
           const int j = r + 16 * h + 32 * (4 * q + i);
-          const float m2 = __builtin_fmaf(zr[i], zr[i], zi[i] * zi[i]);
-          if (j >= cs && j < ce) best = fmaxf(best, m2);
+          if (j >= cs && j < ce) best = max(best, m2);
         }
       }
     }
-    best = wave_max_f32(best);
-    if (lane == 0) {
-      const float inv = (st > 0.f && sd > 0.f) ? 1.0f / (st * sd) : 0.f;
-      cmax2[(long long)d * nt + t] = best * inv * inv;
+    return best;
+  }
+};
+
+__device__ __forceinline__ void load_frag8(cf *ct, const cf *ctm, int t, int lane) {
+  const float4 *src = (const float4 *)(ctm + ((long long)t * 64 + lane) * 8);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const float4 v = src[i];
+    ct[2 * i] = cf{v.x, v.y};
+    ct[2 * i + 1] = cf{v.z, v.w};
+  }
+}
+
+// Each wave holds R data rows, each scaled by an exact power of two below
+// 1/2 in every component, and walks `tpw` templates, fetching the next
+// template's fragments while it transforms the current one; the four waves
+// of a workgroup take 4 R consecutive data rows and the same templates.
+// With |Re|, |Im| < 1/2 for the data and < 1 for the template, |P| < 1,
+// |Y| < 32 and |z| < 512: no fp16 overflow.  Writes the coarse maximum
+// |z_c|^2 over lags [cs, ce) per pair, in units of the unscaled inputs (an
+// exact power-of-two rescale).
+template <bool BF16, int R, bool FULL>
+__global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long long data_stride, const cf *ctm,
+                                                          const int *sexp, int nd, int nt, int tpw, int cs, int ce,
+                                                          const uint32_t *consts, float *cmax2, CoarseGate g) {
+  const int lane = threadIdx.x & 63;
+  const int d0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * R;
+  if (g.zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *g.zero = 0;
+  if (d0 >= nd) return;  // whole waves exit; no barriers in this kernel
+  const int t0 = blockIdx.x * tpw;
+  const int t1 = min(nt, t0 + tpw);
+  const int q = lane >> 4, r = lane & 15;
+
+  // the data rows at the A-fragment positions f = 128 q + 16 i + r; a row
+  // past nd (the wave's last, when nd is not a multiple of R) stays zero
+  cf x[R][8];
+  int sd[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    float dmax = 0.f;
+    const cf *D = data + (long long)min(d0 + k, nd - 1) * data_stride;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      x[k][i] = d0 + k < nd ? D[128 * q + 16 * i + r] : cf{0.f, 0.f};
+      dmax = fmaxf(dmax, fmaxf(fabsf(x[k][i].x), fabsf(x[k][i].y)));
+    }
+    sd[k] = unit_exponent(wave_max_nonneg(dmax)) + 1;  // components below 1/2
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[k][i] = cf{ldexpf(x[k][i].x, -sd[k]), ldexpf(x[k][i].y, -sd[k])};
+  }
+  CoarseDft<BF16> dft;
+  dft.load_table(consts, lane);
+
+  // template t's pairs: the transforms of all R rows first (independent
+  // chains the scheduler interleaves), then the reductions; gate decisions
+  // are kept as wave-uniform bits per row and template (tpw <= 64) and
+  // written out once, after the last template
+  uint64_t fired[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) fired[k] = 0;
+  auto pairs = [&](const cf *ct, int t) {
+    uint32_t lm[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) lm[k] = dft.template lane_max<FULL>(x[k], ct, cs, ce, q, r);
+    const int st = sexp[t];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const float m2 = ldexpf(__uint_as_float(wave_umax(lm[k])), 2 * (st + sd[k]));
+      if (d0 + k >= nd) continue;
+      if (cmax2 && lane == 0) cmax2[(long long)(d0 + k) * nt + t] = m2;
+      if (g.thr2 <= 0.f || m2 >= g.thr2) fired[k] |= 1ull << (t - t0);
+    }
+  };
+
+  // two fragment buffers in turn: the next template's loads are in flight
+  // while the current one is transformed
+  cf ca[8], cb[8];
+  if (t0 < t1) load_frag8(ca, ctm, t0, lane);
+  for (int t = t0; t < t1; t += 2) {
+    if (t + 1 < t1) load_frag8(cb, ctm, t + 1, lane);
+    pairs(ca, t);
+    if (t + 1 >= t1) break;
+    if (t + 2 < t1) load_frag8(ca, ctm, t + 2, lane);
+    pairs(cb, t + 1);
+  }
+  if (!g.list) return;
+
+  // survivors: one atomic per wave reserves their list slots; the other
+  // pairs get -1 / 0 in all their bins, lanes across (template, bin)
+  int total = 0;
+#pragma unroll
+  for (int k = 0; k < R; k++) total += __popcll(fired[k]);
+  int base = 0;
+  if (total) {
+    if (lane == 0) base = atomicAdd(g.count, total);
+    base = __builtin_amdgcn_readfirstlane(base);
+  }
+  const int cnt = t1 - t0;
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    if (d0 + k >= nd) break;
+    const uint64_t f = fired[k];
+    if (lane < cnt && ((f >> lane) & 1ull))
+      g.list[base + __popcll(f & ((1ull << lane) - 1ull))] = int2{d0 + k, t0 + lane};
+    base += __popcll(f);
+    if (__popcll(f) == cnt) continue;
+    const long long o = ((long long)(d0 + k) * nt + t0) * g.nb;
+    for (int e = lane; e < cnt * g.nb; e += 64) {
+      const int j = e / g.nb;
+      if (!((f >> j) & 1ull)) {
+        g.idx[o + e] = -1;
+        g.val[o + e] = cf{0.f, 0.f};
+      }
     }
   }
 }
@@ -237,37 +380,74 @@ __global__ void unit_roots_kernel(cf *out, int L) {
   }
 }
 
+// The per-lane constant table of CoarseDft<BF16>, from the FP64-generated
+// roots; one wave.
+template <bool BF16>
+__global__ void coarse_consts_kernel(const cf *w32, const cf *w512, const cf *w16, uint32_t *tab) {
+  const int lane = threadIdx.x;
+  CoarseDft<BF16> dft;
+  dft.load(w32, w512, w16, lane >> 4, lane & 15);
+  dft.store_table(tab, lane);
+}
+
 }  // namespace
 
 // Roots of unity of orders 32, 512, 16 for the factored DFT, built once.
-static const cf *coarse_roots(Ctx *c) {
+// Then the per-lane constant tables for fp16 (first) and bf16 operands.
+static const uint32_t *coarse_consts(Ctx *c, bool bf16) {
+  constexpr int kWords = 64 * CoarseDft<false>::kLaneWords;
   cf *r = (cf *)c->scratch("coarse_roots", (32 + 512 + 16) * sizeof(cf));
-  if (!r) return nullptr;
+  uint32_t *tab = (uint32_t *)c->scratch("coarse_consts", 2 * kWords * sizeof(uint32_t));
+  if (!r || !tab) return nullptr;
   if (!c->coarse_roots_ready) {
     hipLaunchKernelGGL(unit_roots_kernel, dim3(1), dim3(64), 0, c->stream, r, 32);
     hipLaunchKernelGGL(unit_roots_kernel, dim3(2), dim3(256), 0, c->stream, r + 32, 512);
     hipLaunchKernelGGL(unit_roots_kernel, dim3(1), dim3(64), 0, c->stream, r + 544, 16);
-    if (!hip_ok(hipGetLastError(), "unit_roots_kernel")) return nullptr;
+    hipLaunchKernelGGL(coarse_consts_kernel<false>, dim3(1), dim3(64), 0, c->stream, r, r + 32, r + 544, tab);
+    hipLaunchKernelGGL(coarse_consts_kernel<true>, dim3(1), dim3(64), 0, c->stream, r, r + 32, r + 544,
+                       tab + kWords);
+    if (!hip_ok(hipGetLastError(), "coarse_consts_kernel")) return nullptr;
     c->coarse_roots_ready = true;
   }
-  return r;
+  return tab + (bf16 ? kWords : 0);
 }
 
-int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, const cf *ctmpl, int nd, int nt, int cs, int ce,
-                       bool bf16, float *cmax2, int *zero) {
-  if (c->coarse_band != 512) return fail("the MFMA coarse screen is built for band 512");
-  const cf *roots = coarse_roots(c);
-  if (!roots) return -1;
-  const int rows = std::max(1, c->coarse_rows);
-  dim3 grid((unsigned)((nt + 3) / 4), (unsigned)((nd + rows - 1) / rows));
+int coarse_mfma_prepare(Ctx *c) {
+  if (c->coarse_band != kBand) return 0;
+  cf *ctm = (cf *)c->scratch("coarse_mfma_tmpl", (size_t)c->nt * kBand * sizeof(cf));
+  int *sexp = (int *)c->scratch("coarse_mfma_exp", (size_t)c->nt * sizeof(int));
+  if (!ctm || !sexp) return -1;
+  hipLaunchKernelGGL(coarse_mfma_prep_kernel, dim3((unsigned)c->nt), dim3(64), 0, c->stream, c->coarse_tmpl, ctm,
+                     sexp);
+  if (!hip_ok(hipGetLastError(), "coarse_mfma_prep_kernel")) return -1;
+  c->coarse_ctm = ctm;
+  c->coarse_sexp = sexp;
+  return 0;
+}
+
+int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, int t0, int nd, int nt, int cs, int ce,
+                       bool bf16, float *cmax2, const CoarseGate *gate) {
+  if (c->coarse_band != kBand || !c->coarse_ctm) return fail("the MFMA coarse screen is built for band 512");
+  const uint32_t *consts = coarse_consts(c, bf16);
+  if (!consts) return -1;
+  const int tpw = std::min(64, std::max(1, c->coarse_tpw));
+  const int rows = c->coarse_rpw == 1 ? 1 : 2;
+  dim3 grid((unsigned)((nt + tpw - 1) / tpw), (unsigned)((nd + 4 * rows - 1) / (4 * rows)));
   if (grid.y > 65535) return fail("too many data rows per launch (%d)", nd);
-  Ctx::ProfScope ps(c, "coarse");
+  const cf *ctm = c->coarse_ctm + (size_t)t0 * kBand;
+  const int *sexp = c->coarse_sexp + t0;
+  const bool full = cs == 0 && ce == kBand;  // whole coarse window: no per-lag test
+  decltype(&coarse_mfma_kernel<true, 1, true>) kern;
   if (bf16)
-    hipLaunchKernelGGL(coarse_mfma_kernel<true>, grid, dim3(256), 0, c->stream, data, data_stride, ctmpl, nd, nt, rows,
-                       cs, ce, roots, roots + 32, roots + 544, cmax2, zero);
+    kern = rows == 1 ? (full ? coarse_mfma_kernel<true, 1, true> : coarse_mfma_kernel<true, 1, false>)
+                     : (full ? coarse_mfma_kernel<true, 2, true> : coarse_mfma_kernel<true, 2, false>);
   else
-    hipLaunchKernelGGL(coarse_mfma_kernel<false>, grid, dim3(256), 0, c->stream, data, data_stride, ctmpl, nd, nt,
-                       rows, cs, ce, roots, roots + 32, roots + 544, cmax2, zero);
+    kern = rows == 1 ? (full ? coarse_mfma_kernel<false, 1, true> : coarse_mfma_kernel<false, 1, false>)
+                     : (full ? coarse_mfma_kernel<false, 2, true> : coarse_mfma_kernel<false, 2, false>);
+  Ctx::ProfScope ps(c, "coarse");
+  const CoarseGate g = gate ? *gate : CoarseGate{};
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, c->stream, data, data_stride, ctm, sexp, nd, nt, tpw, cs, ce, consts,
+                     cmax2, g);
   return hip_ok(hipGetLastError(), "coarse_mfma_kernel") ? 0 : -1;
 }
 

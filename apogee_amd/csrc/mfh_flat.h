@@ -366,6 +366,9 @@ flat_kernel(FlatArgs a) {
     for (int u = 0; u < 16; u++) T[u] = valid ? tp[t0 + u * TPT] : cf{0.f, 0.f};
   }
   if (a.zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *a.zero = 0;
+  if constexpr (LIST) {
+    if (dbeg >= dend) return;  // workgroup-uniform: the list is shorter than this slot range
+  }
   // both bin buffers start empty
   for (int i = threadIdx.x; i < 2 * S::SLOTS * nb; i += S::WG) bins_all[i] = 0ull;
   __syncthreads();

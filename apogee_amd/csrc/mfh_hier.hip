@@ -65,8 +65,9 @@ __global__ void __launch_bounds__(256) coarse_template_kernel(const cf *tmpl, lo
 // One atomic per wave: lanes that fire take consecutive slots after the
 // wave's base, found from the ballot.
 __global__ void gate_kernel(const int64_t *cidx, const cf *cval, int nd, int nt, float thr, int nb,
-                            int64_t *out_idx, cf *out_val, int2 *list, int *count) {
+                            int64_t *out_idx, cf *out_val, int2 *list, int *count, int *other) {
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) *other = 0;  // the next call's counter
   const bool inside = p < (long long)nd * nt;
   bool fire = false;
   if (inside && cidx[p] >= 0) {
@@ -74,35 +75,6 @@ __global__ void gate_kernel(const int64_t *cidx, const cf *cval, int nd, int nt,
     const float m2 = __builtin_fmaf(z.x, z.x, z.y * z.y);
     fire = sqrtf(m2) >= thr;
   }
-  const unsigned long long ballot = __ballot(fire);
-  const int lane = threadIdx.x & 63;
-  int base = 0;
-  if (ballot) {
-    const int leader = __ffsll((long long)ballot) - 1;
-    if (lane == leader) base = atomicAdd(count, __popcll(ballot));
-    base = __shfl(base, leader, 64);
-  }
-  if (!inside) return;
-  if (fire) {
-    const int slot = base + __popcll(ballot & ((1ull << lane) - 1ull));
-    list[slot] = int2{(int)(p / nt), (int)(p % nt)};
-  } else {
-    for (int b = 0; b < nb; b++) {
-      out_idx[p * nb + b] = -1;
-      out_val[p * nb + b] = cf{0.f, 0.f};
-    }
-  }
-}
-
-// Gate on an MFMA coarse statistic: fire iff sqrt(cmax2) * (1 + delta) >= thr
-// (every pair when thr == 0), a superset of the FP32 screen's decisions
-// whenever the statistic's relative error is within delta.
-__global__ void gate_max_kernel(const float *cmax2, int nd, int nt, float thr, float delta, int nb,
-                                int64_t *out_idx, cf *out_val, int2 *list, int *count) {
-  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool inside = p < (long long)nd * nt;
-  bool fire = false;
-  if (inside) fire = thr == 0.f || sqrtf(cmax2[p]) * (1.0f + delta) >= thr;
   const unsigned long long ballot = __ballot(fire);
   const int lane = threadIdx.x & 63;
   int base = 0;
@@ -134,8 +106,30 @@ int hier_prepare(Ctx *c) {
                      c->coarse_ref_f, ct);
   if (!hip_ok(hipGetLastError(), "coarse_template_kernel")) return -1;
   c->coarse_tmpl = ct;
+  if (coarse_mfma_prepare(c) < 0) return -1;
   c->coarse_dirty = false;
   return 0;
+}
+
+// The two refine-list counters.  Consecutive calls alternate between them
+// (c->hier_parity): a call's coarse stage appends to its own counter, which
+// the previous call cleared, and clears the other one for the next call --
+// stream order puts that after every read of it -- so no launch of its own
+// clears a counter.  Both start at zero when first allocated (always on an
+// eager call: a graph is captured only on a repeated call).
+int *hier_counter(Ctx *c, int parity) {
+  auto it = c->bufs.find("refine_counters");
+  const bool fresh = it == c->bufs.end() || !it->second.p;
+  int *cnt = (int *)c->scratch("refine_counters", 2 * sizeof(int));
+  if (!cnt) return nullptr;
+  if (fresh) {
+    if (c->capturing) {
+      fail("refine counters allocated during graph capture");
+      return nullptr;
+    }
+    if (!hip_ok(hipMemsetAsync(cnt, 0, 2 * sizeof(int), c->stream), "hipMemsetAsync")) return nullptr;
+  }
+  return cnt + (parity & 1);
 }
 
 int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,
@@ -160,17 +154,29 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   int64_t *cidx = (int64_t *)c->scratch("coarse_idx", pairs * sizeof(int64_t));
   cf *cval = (cf *)c->scratch("coarse_val", pairs * sizeof(cf));
   int2 *list = (int2 *)c->scratch("refine_list", pairs * sizeof(int2));
-  int *count = (int *)c->scratch("refine_count", sizeof(int));
-  if (!cidx || !cval || !list || !count) return -1;
+  int *count = hier_counter(c, c->hier_parity);
+  int *other = hier_counter(c, c->hier_parity ^ 1);
+  if (!cidx || !cval || !list || !count || !other) return -1;
 
   const cf *drows = data_rows ? data_rows : c->data + (size_t)d0 * n;
   const bool mfma = c->coarse_prec > 0 && m == 512;
-  float *cmax2 = nullptr;
   if (mfma) {
-    cmax2 = (float *)c->scratch("coarse_max", pairs * sizeof(float));
-    if (!cmax2) return -1;
-    if (launch_coarse_mfma(c, drows, (long long)n, c->coarse_tmpl + (size_t)t0 * m, nd, nt, (int)cs, (int)ce,
-                           c->coarse_prec == 2, cmax2, count) < 0)
+    // the matrix-core screen gates each pair itself and appends survivors
+    // to this call's counter, which the previous call's screen cleared; it
+    // clears the other one for the next call
+    CoarseGate g;
+    if (coarse_thr > 0.f) {
+      const double w = (double)coarse_thr / (1.0 + (double)c->coarse_delta);
+      g.thr2 = std::nextafter((float)(w * w), 0.f);
+    }
+    g.nb = (int)nb;
+    g.idx = idx;
+    g.val = val;
+    g.list = list;
+    g.count = count;
+    g.zero = other;
+    if (launch_coarse_mfma(c, drows, (long long)n, t0, nd, nt, (int)cs, (int)ce, c->coarse_prec == 2, nullptr, &g) <
+        0)
       return -1;
   }
   FlatRun cr;
@@ -188,18 +194,16 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   cr.out_val = cval;
   cr.prof_name = "coarse";
   cr.zero = count;  // the coarse launch clears the survivor counter
-  if (!mfma && launch_flat(c, m, cr) < 0) return -1;
-
-  {
+  if (!mfma) {
+    if (launch_flat(c, m, cr) < 0) return -1;
     Ctx::ProfScope ps(c, "gate");
-    if (mfma)
-      hipLaunchKernelGGL(gate_max_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, c->stream, cmax2, nd,
-                         nt, coarse_thr, c->coarse_delta, (int)nb, idx, val, list, count);
-    else
-      hipLaunchKernelGGL(gate_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, c->stream, cidx, cval, nd,
-                         nt, coarse_thr, (int)nb, idx, val, list, count);
+    hipLaunchKernelGGL(gate_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, c->stream, cidx, cval, nd,
+                       nt, coarse_thr, (int)nb, idx, val, list, count, other);
+    if (!hip_ok(hipGetLastError(), "gate_kernel")) return -1;
   }
-  if (!hip_ok(hipGetLastError(), "gate_kernel")) return -1;
+  // the screen has cleared the other counter: the next call uses it (a
+  // capture flips the parity when its graph runs)
+  if (!c->capturing) c->hier_parity ^= 1;
 
   FlatRun rr;
   rr.data = drows;
@@ -272,7 +276,7 @@ extern "C" int mfh_hier_coarse_max(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
   if (!cmax2) return -1;
   std::vector<float> m2(pairs);
   if (precision > 0) {
-    if (mfh::launch_coarse_mfma(c, c->data + (size_t)d0 * n, (long long)n, c->coarse_tmpl + (size_t)t0 * m, nd, nt,
+    if (mfh::launch_coarse_mfma(c, c->data + (size_t)d0 * n, (long long)n, t0, nd, nt,
                                 (int)cs, (int)ce, precision == 2, cmax2) < 0 ||
         !hip_ok(hipMemcpyAsync(m2.data(), cmax2, pairs * sizeof(float), hipMemcpyDeviceToHost, c->stream), "copy") ||
         !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
@@ -331,13 +335,16 @@ int hier_run_cached(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, floa
   std::vector<long long> key = {d0, nd, t0, nt, (long long)binsize, fbits(threshold), (long long)ws, (long long)we,
                                 fbits(coarse_thr), (long long)(intptr_t)idx, (long long)(intptr_t)val,
                                 c->coarse_band, dbits(c->coarse_ref_f), c->coarse_prec, fbits(c->coarse_delta),
-                                c->refine_grid_cap, (long long)(intptr_t)c->coarse_tmpl, c->coarse_rows};
-  if (c->graph_exec && key == c->graph_key) {
-    if (!hip_ok(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch")) return -1;
-  } else if (key == c->last_key) {
-    if (c->graph_exec) {
-      hipGraphExecDestroy(c->graph_exec);
-      c->graph_exec = nullptr;
+                                c->refine_grid_cap, (long long)(intptr_t)c->coarse_tmpl, c->coarse_tpw, c->coarse_rpw};
+  const int par = c->hier_parity;
+  if (key == c->graph_key && c->hier_graph[par]) {
+    if (!hip_ok(hipGraphLaunch(c->hier_graph[par], c->stream), "hipGraphLaunch")) return -1;
+    c->hier_parity ^= 1;
+  } else if (key == c->graph_key || key == c->last_key) {
+    // a repeated call: capture the graph of this counter parity
+    if (key != c->graph_key) {
+      c->drop_hier_graphs();
+      c->graph_key = key;
     }
     hipGraph_t g = nullptr;
     if (!hip_ok(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
@@ -354,16 +361,16 @@ int hier_run_cached(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, floa
     const bool inst = hip_ok(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0), "hipGraphInstantiate");
     hipGraphDestroy(g);
     if (!inst) return -1;
-    c->graph_exec = ex;
-    c->graph_key = key;
-    if (!hip_ok(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch")) return -1;
+    c->hier_graph[par] = ex;
+    if (!hip_ok(hipGraphLaunch(ex, c->stream), "hipGraphLaunch")) return -1;
+    c->hier_parity ^= 1;
   } else {
     if (hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, idx, val, nullptr, nullptr) < 0)
       return -1;
     c->last_key = key;
   }
   if (count_out) {
-    int *count = (int *)c->scratch("refine_count", sizeof(int));
+    int *count = hier_counter(c, par);
     if (!count || !hip_ok(hipMemcpyAsync(count_out, count, sizeof(int), hipMemcpyDeviceToHost, c->stream),
                           "copy count"))
       return -1;

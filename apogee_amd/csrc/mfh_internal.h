@@ -32,7 +32,12 @@ struct Ctx {
   // MFMA; the MFMA statistic is widened by (1 + coarse_delta) at the gate
   int coarse_prec = 0;
   float coarse_delta = 0.f;
-  int coarse_rows = 4;  // data rows per wave of the MFMA coarse screen
+  int coarse_tpw = 4;  // templates per wave of the MFMA coarse screen
+  int coarse_rpw = 2;  // data rows per wave of the MFMA coarse screen (1 or 2)
+  // the coarse templates in MFMA fragment order, power-of-two scaled, and
+  // their scale exponents (built with coarse_tmpl at band 512)
+  cf *coarse_ctm = nullptr;
+  int *coarse_sexp = nullptr;
   bool coarse_roots_ready = false;
   // four-step lengths (32768, 65536): roots built, scratch budget per chunk
   bool big_roots_ready[16] = {};  // by log2 of the column length
@@ -66,6 +71,17 @@ struct Ctx {
   // identical call, so every buffer and kernel attribute exists by then).
   bool use_graphs = true;
   hipGraphExec_t graph_exec = nullptr;
+  // the hierarchical sequence's graphs, one per refine-counter parity, both
+  // under graph_key; hier_parity is the counter the next call uses
+  hipGraphExec_t hier_graph[2] = {nullptr, nullptr};
+  int hier_parity = 0;
+  void drop_hier_graphs() {
+    for (auto &g : hier_graph)
+      if (g) {
+        hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+  }
   std::vector<long long> graph_key, last_key;
   std::map<std::string, std::pair<double, long>> prof_totals;
   void prof_collect();
@@ -117,10 +133,27 @@ const cf *big_roots(Ctx *c, int n1);
 int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out);
 // MFMA coarse statistic (max |z_c|^2 over [cs, ce)) per pair, band 512;
 // `zero` (may be null) is set to 0 by the launch.
-int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, const cf *ctmpl, int nd, int nt, int cs, int ce,
-                       bool bf16, float *cmax2, int *zero = nullptr);
+int coarse_mfma_prepare(Ctx *c);
+// The gate applied inside the coarse screen when `list` is set: a pair
+// fires iff thr2 <= 0 or |z_c|^2 >= thr2, with thr2 = (thr / (1 + delta))^2
+// rounded down (a superset of the FP32 screen's decisions whenever the
+// statistic's relative error is within delta); fired pairs are appended to
+// `list` (count must be zero at launch), the others get -1 / 0 in their nb
+// output bins.
+struct CoarseGate {
+  float thr2 = 0.f;
+  int nb = 0;
+  int64_t *idx = nullptr;
+  cf *val = nullptr;
+  int2 *list = nullptr;
+  int *count = nullptr;
+  int *zero = nullptr;  // set to 0 by the launch (the next call's counter)
+};
+int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, int t0, int nd, int nt, int cs, int ce,
+                       bool bf16, float *cmax2, const CoarseGate *gate = nullptr);
 // Hierarchical run over data rows [d0, d0+nd) of the context, or over
 // `data_rows` (stride n) when not null.
+int *hier_counter(Ctx *c, int parity);
 int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,
              float coarse_thr, int64_t *idx, cf *val, int *count_out, const cf *data_rows);
 

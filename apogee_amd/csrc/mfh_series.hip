@@ -239,13 +239,10 @@ long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int s
     cf *ov = dval + (size_t)b0 * nt * nb;
     if (hier) {
       refined.push_back(0);
-      if (mfh::hier_run(c, 0, g, t0, nt, binsize, threshold, ws, we, coarse_thr, oi, ov, nullptr,
-                        spec + (size_t)b0 * n) < 0)
-        return -1;
       // survivor counts are read from the device list counter per group
       int cnt = 0;
-      int *dcount = (int *)c->scratch("refine_count", sizeof(int));
-      if (!hip_ok(hipMemcpyAsync(&cnt, dcount, sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy count") ||
+      if (mfh::hier_run(c, 0, g, t0, nt, binsize, threshold, ws, we, coarse_thr, oi, ov, &cnt,
+                        spec + (size_t)b0 * n) < 0 ||
           !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
         return -1;
       refined_total += cnt;

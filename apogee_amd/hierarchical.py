@@ -74,6 +74,7 @@ class HierarchicalFilter(MatchedFilter):
         from . import autotune
         autotune.apply(self._gpu, "flat", self.n, self.ndata, self.ntemplates)
         autotune.apply(self._gpu, "hier", self.n, self.ndata, self.ntemplates)
+        autotune.apply(self._gpu, "coarse", self.n, self.ndata, self.ntemplates)   # the band-512 MFMA screen
 
     # ----------------------------------------------------------- settings
     def set_reference(self, power):

@@ -132,15 +132,20 @@ def hier_leg(args):
     val = torch.empty((ND, NT, 1), dtype=torch.complex64, device="cuda")
 
     def timed(fn, ctx, steps):
-        """Wall time per call with profiling off, then per-stage event times."""
-        for _ in range(5):
+        """Wall time per call with profiling off (the best of three runs of
+        `steps` back-to-back calls, after 50 untimed ones: the host's launch
+        path settles over the first few dozen calls), then per-stage event
+        times."""
+        for _ in range(50):
             fn()
         ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn()
-        ctx.sync()
-        dt = (time.perf_counter() - t0) / steps
+        dt = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            ctx.sync()
+            dt = min(dt, (time.perf_counter() - t0) / steps)
         ctx.prof_enable(True)
         for _ in range(max(5, steps // 4)):
             fn()

@@ -206,8 +206,9 @@ long mfh_hier_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, i
                          int64_t *n_refined);
 
 /* Tuning knobs: key "dpw" (data rows per workgroup; 0 = automatic),
- * "refine_grid" (workgroups of a list-driven refine), "coarse_rows" (data
- * rows per wave of the matrix-core coarse screen), "big_chunk_mb" (scratch
+ * "refine_grid" (workgroups of a list-driven refine), "coarse_tpw"
+ * (templates per wave of the matrix-core coarse screen, 1..64), "coarse_rpw"
+ * (its data rows per wave, 1 or 2), "big_chunk_mb" (scratch
  * per four-step launch pair at n = 32768 / 65536), "graphs" (0/1).
  * Replaces: MF_MFTILE / MF_PBMAX environment knobs (src/matchfilt.c:93-107). */
 int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value);

@@ -281,9 +281,10 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
   const int lane = threadIdx.x & 63;
   const int d0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * R;
   if (g.zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *g.zero = 0;
-  if (d0 >= nd) return;  // whole waves exit; no barriers in this kernel
+  // a wave past the last data row walks no templates but stays for the
+  // workgroup's barriers at the end
   const int t0 = blockIdx.x * tpw;
-  const int t1 = min(nt, t0 + tpw);
+  const int t1 = d0 < nd ? min(nt, t0 + tpw) : t0;
   const int q = lane >> 4, r = lane & 15;
 
   // the data rows at the A-fragment positions f = 128 q + 16 i + r; a row
@@ -338,18 +339,25 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
     if (t + 2 < t1) load_frag8(ca, ctm, t + 2, lane);
     pairs(cb, t + 1);
   }
-  if (!g.list) return;
+  if (!g.list) return;  // uniform over the grid
 
-  // survivors: one atomic per wave reserves their list slots; the other
+  // survivors: one atomic per workgroup reserves their list slots (a single
+  // counter takes every workgroup's atomic, so fewer is faster); the other
   // pairs get -1 / 0 in all their bins, lanes across (template, bin)
+  __shared__ int wave_cnt[4], wg_base;
+  const int w = threadIdx.x >> 6;
   int total = 0;
 #pragma unroll
   for (int k = 0; k < R; k++) total += __popcll(fired[k]);
-  int base = 0;
-  if (total) {
-    if (lane == 0) base = atomicAdd(g.count, total);
-    base = __builtin_amdgcn_readfirstlane(base);
+  if (lane == 0) wave_cnt[w] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int sum = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+    wg_base = sum ? atomicAdd(g.count, sum) : 0;
   }
+  __syncthreads();
+  int base = wg_base;
+  for (int i = 0; i < w; i++) base += wave_cnt[i];
   const int cnt = t1 - t0;
 #pragma unroll
   for (int k = 0; k < R; k++) {

@@ -201,3 +201,49 @@ def test_sub_ranges_match_slices(gpu_device):
     sub = hier.run(binsize=512, threshold=5.0, data=(2, 3), templates=(1, 3))
     np.testing.assert_array_equal(sub["index"], full["index"][2:5, 1:4])
     np.testing.assert_array_equal(sub["value"], full["value"][2:5, 1:4])
+
+
+def test_repeated_calls_alternate_counters_and_graphs(gpu_device):
+    """Consecutive calls alternate two survivor counters and identical calls
+    replay one captured graph per counter parity.  A run of identical calls,
+    new data between replays and switches of the coarse precision must each
+    give exactly what a fresh filter gives on the same inputs, survivor
+    counts included."""
+    n, nd, nt = 4096, 16, 64
+    rng = np.random.default_rng(23)
+    power = inspiral_power(n)
+    H = np.stack([template_with_power(n, power, rng) for _ in range(nt)])
+    datasets = []
+    for s in range(2):
+        D = noise((nd, n), rng)
+        for d in range(0, nd, 2 + s):     # different survivor sets per data set
+            lag = 100 + 31 * d + 7 * s
+            D[d] += (9.0 * H[(5 * d + s) % nt] * np.exp(2j * np.pi * np.arange(n) * lag / n)).astype(np.complex64)
+        datasets.append(D)
+
+    def fresh(prec, D):
+        h = mf.HierarchicalFilter(n, nd, nt, snr=5.5, fd=1e-2, band=512, device="gpu")
+        h.set_reference(power)
+        h.set_coarse_precision(prec)
+        h.set_templates(H)
+        h.set_data(D)
+        out = h.run(binsize=1024, threshold=5.0)
+        return out["index"].copy(), out["value"].copy(), h.stats[1]
+
+    expected = {(p, i): fresh(p, D) for p in ("fp16", "bf16", "fp32") for i, D in enumerate(datasets)}
+    h = mf.HierarchicalFilter(n, nd, nt, snr=5.5, fd=1e-2, band=512, device="gpu")
+    h.set_reference(power)
+    h.set_templates(H)
+    steps = [("fp16", 0)] * 5 + [("fp16", 1)] * 3 + [("fp32", 0)] * 3 + [("bf16", 1)] * 4 + \
+            [("fp16", 0), ("bf16", 0), ("fp16", 1)] * 2
+    for step, (prec, i) in enumerate(steps):
+        h.set_coarse_precision(prec)
+        h.set_data(datasets[i])
+        before = h.stats[1]
+        out = h.run(binsize=1024, threshold=5.0)
+        idx, val, refined = expected[(prec, i)]
+        ctx = "step %d: %s on data set %d" % (step, prec, i)
+        np.testing.assert_array_equal(out["index"], idx, err_msg=ctx)
+        np.testing.assert_array_equal(out["value"], val, err_msg=ctx)
+        assert h.stats[1] - before == refined, ctx
+        assert 0 < refined < nd * nt, ctx

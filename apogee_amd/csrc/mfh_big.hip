@@ -32,7 +32,7 @@ namespace {
 struct BigArgs {
   const cf *data, *tmpl;
   long long data_stride, tmpl_stride;
-  int nd, nt;           // tiled mode: pair p = p0 + blockIdx -> (p / nt, p % nt)
+  int nd, nt;           // tiled mode: pair p = p0 + blockIdx -> (d, t) = (p % nd, p / nd)
   long long p0;         // first pair (tiled) or list position (list) of this chunk
   const int2 *list;     // list mode
   const int *list_count;
@@ -59,8 +59,10 @@ __device__ __forceinline__ bool big_pair(const BigArgs &a, long long p, int &d, 
     return true;
   }
   if (p >= (long long)a.nd * a.nt) return false;
-  d = (int)(p / a.nt);
-  t = (int)(p % a.nt);
+  // template-major: the nd pairs of one template are neighbours, so its
+  // spectrum is re-read from L2 rather than HBM (2.5% at n = 65536, 4 x 128)
+  t = (int)(p / a.nd);
+  d = (int)(p % a.nd);
   return true;
 }
 

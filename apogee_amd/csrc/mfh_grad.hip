@@ -55,10 +55,13 @@ __device__ __forceinline__ cf circle(const cf *lds, int n, uint32_t m) {
   return cmul(lds[m >> 8], lds[nh + (m & 255u)]);
 }
 
-// dL/dT (TEMPL = true) or dL/dD: one thread per (row, f); the row is
-// blockIdx.y, the other side's rows are summed over.  The row's (lag,
-// gradient) list is staged through LDS in chunks, so the inner loop reads
-// LDS instead of waiting on one global load per term.
+// dL/dT (TEMPL = true) or dL/dD: one workgroup per (row, 64 consecutive f);
+// the row is blockIdx.y, the other side's rows are summed over.  The row's
+// (lag, gradient) list is staged through LDS in chunks, so the inner loop
+// reads LDS instead of waiting on one global load per term, and the four
+// waves of the workgroup take every fourth entry of a chunk: four times the
+// waves in flight of one thread per (row, f), then a fixed-order sum of the
+// four partial sums (deterministic).
 template <bool TEMPL>
 __global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ other, long long other_stride,
                                                         int nother, const int64_t *__restrict__ idx,
@@ -69,10 +72,11 @@ __global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ o
   __shared__ cf roots[512];
   __shared__ uint32_t sk[CH];
   __shared__ cf sg[CH];
+  __shared__ cf part[4][64];
   stage_circle(tables, n, roots);
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int f = blockIdx.x * 64 + lane;  // n is a multiple of 64
   const int row = blockIdx.y;
-  const bool valid = f < n;
   const uint32_t mask = (uint32_t)n - 1u;
   float ax = 0.f, ay = 0.f;
   const long long total = (long long)nother * nb;
@@ -88,38 +92,38 @@ __global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ o
       sg[i] = gval[pair * nb + b];
     }
     __syncthreads();
-    if (valid) {
-      int j = (int)(e0 / nb), b = (int)(e0 % nb), jx = -1;
-      cf x = cf{0.f, 0.f};
-      for (int i = 0; i < cnt; i++) {
-        const uint32_t k = sk[i];
-        if (k != 0xffffffffu) {
-          if (j != jx) {
-            x = other[(long long)j * other_stride + f];
-            jx = j;
-          }
-          const cf g = sg[i];
-          const cf w = circle(roots, n, ((uint32_t)f * k) & mask);
-          // templates: conj(g) x w ; data: g x conj(w)
-          cf xw;
-          if (TEMPL) {
-            xw = cmul(x, w);
-            ax += g.x * xw.x + g.y * xw.y;
-            ay += g.x * xw.y - g.y * xw.x;
-          } else {
-            xw = cmul_conj(x, w);
-            ax += g.x * xw.x - g.y * xw.y;
-            ay += g.x * xw.y + g.y * xw.x;
-          }
+    int jx = -1;
+    cf x = cf{0.f, 0.f};
+    for (int i = w; i < cnt; i += 4) {
+      const uint32_t k = sk[i];
+      if (k != 0xffffffffu) {
+        const int j = (int)((e0 + i) / nb);
+        if (j != jx) {
+          x = other[(long long)j * other_stride + f];
+          jx = j;
         }
-        if (++b == nb) {
-          b = 0;
-          j++;
+        const cf g = sg[i];
+        const cf wk = circle(roots, n, ((uint32_t)f * k) & mask);
+        // templates: conj(g) x w ; data: g x conj(w)
+        cf xw;
+        if (TEMPL) {
+          xw = cmul(x, wk);
+          ax += g.x * xw.x + g.y * xw.y;
+          ay += g.x * xw.y - g.y * xw.x;
+        } else {
+          xw = cmul_conj(x, wk);
+          ax += g.x * xw.x - g.y * xw.y;
+          ay += g.x * xw.y + g.y * xw.x;
         }
       }
     }
   }
-  if (valid) out[(long long)row * out_stride + f] = cf{ax, ay};
+  part[w][lane] = cf{ax, ay};
+  __syncthreads();
+  if (w == 0) {
+    const cf s0 = part[0][lane], s1 = part[1][lane], s2 = part[2][lane], s3 = part[3][lane];
+    out[(long long)row * out_stride + f] = cf{(s0.x + s1.x) + (s2.x + s3.x), (s0.y + s1.y) + (s2.y + s3.y)};
+  }
 }
 
 // z at stored lags: one wave per (pair, bin); FP32 products, FP64 sums.
@@ -187,20 +191,20 @@ extern "C" int mfh_peak_grad_async(mfh_ctx *ctx, const void *dev_data_c64, long 
   if (!dev_data_c64 || !dev_tmpl_c64 || !dev_idx || !dev_gval_c64) return fail("null input pointer");
   if (data_stride < (long long)c->n || tmpl_stride < (long long)c->n) return fail("row stride below n");
   if (nd > 65535 || nt > 65535) return fail("at most 65535 data and template rows per call");
-  if (c->n > 65536) return fail("peak gradients are built for n <= 65536");
+  if (c->n > 65536 || c->n < 64) return fail("peak gradients are built for 64 <= n <= 65536");
   hipSetDevice(c->dev);
   const cf *roots = mfh::unit_circle(c);
   if (!roots) return -1;
   const int n = (int)c->n;
   const dim3 block(256);
   if (dev_grad_tmpl_c64) {
-    hipLaunchKernelGGL(mfh::peak_grad_kernel<true>, dim3((n + 255) / 256, nt), block, 0, c->stream,
+    hipLaunchKernelGGL(mfh::peak_grad_kernel<true>, dim3(n / 64, nt), block, 0, c->stream,
                        (const cf *)dev_data_c64, data_stride, nd, dev_idx, (const cf *)dev_gval_c64, nd, nt, (int)nb,
                        roots, n, (cf *)dev_grad_tmpl_c64, (long long)n);
     if (!hip_ok(hipGetLastError(), "peak_grad_kernel")) return -1;
   }
   if (dev_grad_data_c64) {
-    hipLaunchKernelGGL(mfh::peak_grad_kernel<false>, dim3((n + 255) / 256, nd), block, 0, c->stream,
+    hipLaunchKernelGGL(mfh::peak_grad_kernel<false>, dim3(n / 64, nd), block, 0, c->stream,
                        (const cf *)dev_tmpl_c64, tmpl_stride, nt, dev_idx, (const cf *)dev_gval_c64, nd, nt, (int)nb,
                        roots, n, (cf *)dev_grad_data_c64, (long long)n);
     if (!hip_ok(hipGetLastError(), "peak_grad_kernel")) return -1;

@@ -203,6 +203,14 @@ def hier_leg(args):
             "fd_omitted": omitted, "fd_injected": injected,
             "stage_us_event_timed": {"coarse": st_h["coarse"], "gate": st_h["gate"],
                                      "refine": st_h["refine"]}}
+        if prec != "fp32" and st_h["coarse"] > 0:
+            # the factored 512-point DFT per pair: 8 MFMA 16x16x32 (16384 flop
+            # each) + 8 MFMA 16x16x16 (8192 each); dense fp16/bf16 peak 2.5 PF
+            flop = pairs * (8 * 16384 + 8 * 8192)
+            ach = flop / (st_h["coarse"] * 1e-6) / 1e12
+            out["screens"][prec]["coarse_roofline"] = {
+                "bound": "mfma", "achieved": ach, "peak": 2500.0, "unit": "TFLOP/s", "frac": ach / 2500.0,
+                "algorithmic_flop_per_launch": flop, "kernel_us": st_h["coarse"]}
     best = out["screens"]["fp16"]
     out["hier_pairs_per_s"] = best["hier_pairs_per_s"]
     out["hier_over_flat"] = best["hier_over_flat"]

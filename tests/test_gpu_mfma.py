@@ -101,3 +101,28 @@ def test_mfma_screen_keeps_the_fp32_survivors(prec, gpu_device):
     assert not ((b32["index"] >= 0) & (bm["index"] < 0)).any()
     assert hm.stats[1] >= h32.stats[1]
     assert hm.refine_rate < 0.8
+
+
+@pytest.mark.parametrize("rpw,tpw", [(1, 1), (1, 3), (1, 8), (2, 2), (2, 5), (2, 16), (1, 64), (2, 64)])
+def test_every_coarse_launch_shape_stays_within_the_widening(rpw, tpw, gpu_device):
+    """The rows and templates per wave (autotuned) change only the work split:
+    every shape keeps the statistic within half the widening of the FP32
+    screen, including ragged edges (nd = 7 rows, nt = 101 templates) and a
+    sub-range of templates."""
+    n, nd, nt = 4096, 7, 101
+    rng = np.random.default_rng(40 + rpw * 100 + tpw)
+    T = bank("inspiral", rng, n, nt)
+    D = (rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))).astype(np.complex64)
+    D[3] += (15 * T[9] * np.sqrt(n) * np.linalg.norm(D[3]) / np.sqrt(n)).astype(np.complex64)
+    h = mf.HierarchicalFilter(n, nd, nt, band=512, device="gpu")
+    h.set_templates(T)
+    h.set_data(D)
+    h._gpu.set_tuning("coarse_rpw", rpw)
+    h._gpu.set_tuning("coarse_tpw", tpw)
+    for rng_t in (None, (13, 77)):
+        ref = h.coarse_statistic("fp32", templates=rng_t).astype(np.float64)
+        for prec in ("fp16", "bf16"):
+            got = h.coarse_statistic(prec, templates=rng_t).astype(np.float64)
+            assert got.shape == ref.shape
+            rel = np.abs(got - ref) / ref
+            assert rel.max() <= _COARSE_PRECISIONS[prec][1] / 2, (prec, rng_t, rel.max())

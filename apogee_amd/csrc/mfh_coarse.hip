@@ -27,6 +27,7 @@
 
 #include "mfh_internal.h"
 #include "mfh_mfma.h"
+#include "mfh_wave.h"
 #include "mfhip.h"
 
 namespace mfh {
@@ -89,28 +90,7 @@ struct Frag<true> {
   }
 };
 
-// Max over the wave of nonnegative floats, given and returned as their bit
-// patterns (which order like the values), every lane receiving it: four DPP
-// steps leave each row of 16 lanes holding its row maximum (xor 1 and xor 2
-// by quad permutes, then the mirrored half row and the mirrored row), and
-// the four row maxima are read into scalars.
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp_umax(uint32_t v) {
-  return max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false));
-}
-
-__device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
-  v = dpp_umax<0xB1>(v);   // quad_perm(1, 0, 3, 2)
-  v = dpp_umax<0x4E>(v);   // quad_perm(2, 3, 0, 1)
-  v = dpp_umax<0x141>(v);  // row_half_mirror
-  v = dpp_umax<0x140>(v);  // row_mirror
-  const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
-  const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
-  const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
-  const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
-  return max(max(a, b), max(c, d));
-}
-
+// The wave maximum of a nonnegative float, every lane receiving it.
 __device__ __forceinline__ float wave_max_nonneg(float v) { return __uint_as_float(wave_umax(__float_as_uint(v))); }
 
 // Exponent s with m * 2^-s < 1 and m * 2^-s >= 1/2 for a finite m > 0 (the

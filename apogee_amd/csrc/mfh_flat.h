@@ -26,6 +26,7 @@
 // writes index and value, so the value always belongs to the reported lag.
 #pragma once
 #include "mfh_dft.h"
+#include "mfh_wave.h"
 
 namespace mfh {
 
@@ -180,6 +181,32 @@ __device__ __forceinline__ bool full_window_bins(const cf *v, int t, int lane, f
   constexpr int LT = ilog2(TPT);
   if (bshift - LT != 1) return false;
   runs8_max_to_bins<TPT>(v, (uint32_t)t, (uint32_t)(t - lane), lane, seed, bshift, bins);
+  return true;
+}
+
+// One bin over the whole transform, TPT >= 64 (a slot spans whole waves):
+// the wave's key (bits(m2) << 32) | ~lag of its largest m2 > seed, lowest lag
+// first -- the key the per-sample scan builds -- from 32-bit operations only:
+// each lane's largest m2 on the bit patterns (nonnegative floats order like
+// their bits) and its lowest u, the wave maximum, then the lowest lag among
+// the lanes that hold it.  Returns false when the wave holds a NaN (bits
+// above +inf): the caller then runs the per-sample scan, which never selects
+// one.
+template <int TPT>
+__device__ __forceinline__ bool one_bin_wave_key(const cf *v, int t, float seed, uint64_t &wkey) {
+  uint32_t mb[16];
+#pragma unroll
+  for (int u = 0; u < 16; u++) mb[u] = __float_as_uint(__builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y));
+  uint32_t m = mb[0];
+#pragma unroll
+  for (int u = 1; u < 16; u++) m = max(m, mb[u]);
+  int bu = 15;
+#pragma unroll
+  for (int u = 14; u >= 0; u--) bu = mb[u] == m ? u : bu;
+  const uint32_t mw = wave_umax(m);
+  if (mw > 0x7f800000u) return false;  // wave-uniform
+  const uint32_t kw = wave_umin(m == mw ? (uint32_t)(t + bu * TPT) : 0xffffffffu);
+  wkey = mw > __float_as_uint(seed) ? ((uint64_t)mw << 32) | (uint64_t)(~kw) : 0ull;
   return true;
 }
 
@@ -413,48 +440,46 @@ flat_kernel(FlatArgs a) {
     inverse_fft<N>(v, t, xch, a.twiddle);
 
     if constexpr (MODE == kOneBin) {
-      // ---- one bin: thread-local argmax, one wave reduction, one LDS max ----
-      float bm = a.seed;
-      int bu = -1;
-      cf bz = cf{0.f, 0.f};
-      if (a.ws == 0 && a.we == (uint32_t)N) {
-        // the whole transform: no per-lag window test (uniform branch)
-#pragma unroll
-        for (int u = 0; u < 16; u++) {
-          const float m2 = __builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y);
-          if (m2 > bm) {  // strict: the lowest lag keeps a tie
-            bm = m2;
-            bu = u;
-            bz = v[u];
-          }
-        }
-      } else {
+      // ---- one bin: a wave key, one LDS max; the winning lag's owner writes ----
+      uint64_t wkey = 0ull;
+      bool have = false;
+      if constexpr (TPT >= 64) {
+        if (a.ws == 0 && a.we == (uint32_t)N) have = one_bin_wave_key<TPT>(v, t, a.seed, wkey);
+      }
+      if (!have) {
+        // per-sample scan: strict, so the lowest lag keeps a tie
+        float bm = a.seed;
+        int bu = -1;
 #pragma unroll
         for (int u = 0; u < 16; u++) {
           const uint32_t k = (uint32_t)(t + u * TPT);
           const float m2 = __builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y);
-          if (k >= a.ws && k < a.we && m2 > bm) {  // strict: the lowest lag keeps a tie
+          if (k >= a.ws && k < a.we && m2 > bm) {
             bm = m2;
             bu = u;
-            bz = v[u];
           }
         }
+        const uint64_t key =
+            bu >= 0 ? ((uint64_t)__float_as_uint(bm) << 32) | (uint64_t)(~(uint32_t)(t + bu * TPT)) : 0ull;
+        if constexpr (TPT >= 64) wkey = wave_max_u64(key);
+        else wkey = key;  // several slots per wave: every lane raises its own key
       }
-      const uint32_t kb = (uint32_t)(t + bu * TPT);
-      const uint64_t key = bu >= 0 ? ((uint64_t)__float_as_uint(bm) << 32) | (uint64_t)(~kb) : 0ull;
-      if constexpr (TPT >= 64) {
-        const uint64_t w = wave_max_u64(key);
-        if (lane == 0 && w) atomicMax((unsigned long long *)&bins[0], (unsigned long long)w);
-      } else {
-        if (key) atomicMax((unsigned long long *)&bins[0], (unsigned long long)key);
-      }
+      if (TPT >= 64 ? (lane == 0 && wkey) : wkey != 0ull)
+        atomicMax((unsigned long long *)&bins[0], (unsigned long long)wkey);
       __syncthreads();
       if (valid) {
         const long long o = (LIST && a.list_ooff) ? a.list_ooff[dd + slot] : ((long long)d * a.nt + tt) * a.ostride;
         const uint64_t win = bins[0];
-        if (key && key == win) {
-          a.out_idx[o] = (int64_t)kb;
-          a.out_val[o] = bz;
+        const uint32_t kwin = ~(uint32_t)win;
+        if (win && (int)(kwin % TPT) == t) {
+          // lag kwin = t + uw * TPT is this thread's sample uw
+          const int uw = (int)(kwin / TPT);
+          cf z = v[0];
+#pragma unroll
+          for (int u = 1; u < 16; u++)
+            if (u == uw) z = v[u];
+          a.out_idx[o] = (int64_t)kwin;
+          a.out_val[o] = z;
         }
         if (t == 0 && win == 0ull) {
           a.out_idx[o] = -1;

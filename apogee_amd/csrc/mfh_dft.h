@@ -81,22 +81,55 @@ __device__ __forceinline__ void dft8(cf *v) {
   for (int i = 0; i < 8; i++) v[i] = o[i];
 }
 
+// The second-layer 4-point DFT of a 16-point transform whose third input
+// still owes the factor w8 = exp(+i pi/4) (SIGN = +1) or w8^3 (SIGN = -1):
+// a2 = b2 * w8^(2 - SIGN).  The constant multiply folds into the sum and the
+// difference with a0 as two FMAs per component, where the separate rotation
+// costs a multiply and an add each.
+template <int SIGN>
+__device__ __forceinline__ void dft4_w8_third(cf &a0, cf &a1, cf &b2, cf &a3) {
+  // b2 * w8 = ((x - y) k, (x + y) k); b2 * w8^3 = (-(x + y) k, (x - y) k)
+  const float p = SIGN > 0 ? b2.x - b2.y : -(b2.x + b2.y);
+  const float q = SIGN > 0 ? b2.x + b2.y : b2.x - b2.y;
+  const cf s0 = {__builtin_fmaf(p, kSqrtHalf, a0.x), __builtin_fmaf(q, kSqrtHalf, a0.y)};
+  const cf d0 = {__builtin_fmaf(-p, kSqrtHalf, a0.x), __builtin_fmaf(-q, kSqrtHalf, a0.y)};
+  const cf s1 = cadd(a1, a3), d1 = cmul_i(csub(a1, a3));
+  a0 = cadd(s0, s1);
+  b2 = csub(s0, s1);
+  a1 = cadd(d0, d1);
+  a3 = csub(d0, d1);
+}
+
+// The second-layer 4-point DFT for k1 = 2, inputs owing 1, w8, i, w8^3:
+// with u = b1 + i b3 and v = b1 - i b3, a1 + a3 = w8 u and
+// i (a1 - a3) = w8^3 v, and both rotations fold into the output FMAs.
+__device__ __forceinline__ void dft4_k2(cf &a0, cf &b1, cf &b2, cf &b3) {
+  const cf s0 = {a0.x - b2.y, a0.y + b2.x};  // a0 + i b2
+  const cf d0 = {a0.x + b2.y, a0.y - b2.x};  // a0 - i b2
+  const cf u = {b1.x - b3.y, b1.y + b3.x};
+  const cf v = {b1.x + b3.y, b1.y - b3.x};
+  const float pu = u.x - u.y, qu = u.x + u.y;  // w8 u = (pu k, qu k)
+  const float pv = v.x - v.y, qv = v.x + v.y;  // w8^3 v = (-qv k, pv k)
+  a0 = cf{__builtin_fmaf(pu, kSqrtHalf, s0.x), __builtin_fmaf(qu, kSqrtHalf, s0.y)};
+  b2 = cf{__builtin_fmaf(-pu, kSqrtHalf, s0.x), __builtin_fmaf(-qu, kSqrtHalf, s0.y)};
+  b1 = cf{__builtin_fmaf(-qv, kSqrtHalf, d0.x), __builtin_fmaf(pv, kSqrtHalf, d0.y)};
+  b3 = cf{__builtin_fmaf(qv, kSqrtHalf, d0.x), __builtin_fmaf(-pv, kSqrtHalf, d0.y)};
+}
+
 // 16-point inverse DFT as 4 x 4: x[4 n1 + n2].
 __device__ __forceinline__ void dft16(cf *v) {
 #pragma unroll
   for (int n2 = 0; n2 < 4; n2++) dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
-  // Y[n2][k1] at v[4 k1 + n2] *= w16^(n2 k1)
+  // Y[n2][k1] at v[4 k1 + n2] owes w16^(n2 k1); the w8 powers (n2 k1 = 2,
+  // 6) and i (n2 k1 = 4) are folded into the second layer's arithmetic
   v[5] = rot_w16_1(v[5]);
-  v[6] = rot_w8_1(v[6]);
   v[7] = rot_w16_3(v[7]);
-  v[9] = rot_w8_1(v[9]);
-  v[10] = cmul_i(v[10]);
-  v[11] = rot_w8_3(v[11]);
   v[13] = rot_w16_3(v[13]);
-  v[14] = rot_w8_3(v[14]);
   v[15] = rot_w16_9(v[15]);
-#pragma unroll
-  for (int k1 = 0; k1 < 4; k1++) dft4(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+  dft4(v[0], v[1], v[2], v[3]);
+  dft4_w8_third<+1>(v[4], v[5], v[6], v[7]);
+  dft4_k2(v[8], v[9], v[10], v[11]);
+  dft4_w8_third<-1>(v[12], v[13], v[14], v[15]);
   // X[k1 + 4 k2] sits at v[4 k1 + k2]
   cf o[16];
 #pragma unroll

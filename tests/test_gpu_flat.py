@@ -305,6 +305,37 @@ def test_nan_is_contained_to_its_segment(gpu_device):
             np.testing.assert_array_equal(got["value"][i], ref["value"][i])
 
 
+@pytest.mark.parametrize("n", [1024, 4096, 16384])
+def test_one_bin_nan_and_infinity(n, gpu_device):
+    """One bin over the whole transform (the wave-reduction selection): a
+    NaN spectrum turns every lag of its pairs into NaN, which never wins, so
+    those pairs report -1 / 0 and every other pair is untouched; an infinite
+    sample is allowed to win, and then only a lag whose |z|^2 is +inf."""
+    nd, nt = 4, 3
+    rng = np.random.default_rng(61)
+    D, T = cnoise(rng, (nd, n)), cnoise(rng, (nt, n))
+    ref = run(D, T, None, 0.0)
+    bad = D.copy()
+    bad[2, 17] = np.complex64(complex(np.nan, 0.0))
+    got = run(bad, T, None, 0.0)
+    assert (got["index"][2] == -1).all() and (got["value"][2] == 0).all()
+    for i in range(nd):
+        if i != 2:
+            np.testing.assert_array_equal(got["index"][i], ref["index"][i])
+            np.testing.assert_array_equal(got["value"][i], ref["value"][i])
+    big = D.copy()
+    big[1, 0] = np.complex64(complex(np.inf, 0.0))   # f = 0: z[k] += inf for every k
+    got = run(big, T, None, 0.0)
+    for j in range(nt):
+        k = got["index"][1, j, 0]
+        if k >= 0:
+            z = got["value"][1, j, 0]
+            assert np.isinf(np.float32(z.real) ** 2 + np.float32(z.imag) ** 2), (j, k, z)
+    for i in range(nd):
+        if i != 1:
+            np.testing.assert_array_equal(got["index"][i], ref["index"][i])
+
+
 def test_window_subrange_raw_and_counts(gpu_device):
     n, nd, nt = 4096, 4, 3
     rng = np.random.default_rng(3)

@@ -504,7 +504,7 @@ static int queue_run_cached(Queue *q, int64_t *idx, cf *val) {
   Ctx *c = &q->c;
   if (!c->use_graphs || c->prof_on) return queue_launch(q, idx, val);
   std::vector<long long> key = {q->plan_gen, (long long)(intptr_t)q->pool, (long long)(intptr_t)idx,
-                                (long long)(intptr_t)val, q->grid_per_cu, q->grid_cap};
+                                (long long)(intptr_t)val, q->grid_per_cu, q->grid_cap, q->c.big_chunk_bytes};
   if (c->graph_exec && key == c->graph_key) {
     return hip_ok(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch") ? 0 : -1;
   }
@@ -597,6 +597,11 @@ extern "C" int mfh_queue_set_tuning(mfh_queue *queue, const char *key, long valu
   }
   if (!strcmp(key, "graphs")) {
     q->c.use_graphs = value != 0;
+    return 0;
+  }
+  if (!strcmp(key, "big_chunk_mb")) {
+    if (value < 1 || value > 65536) return fail("big_chunk_mb must be in [1, 65536]");
+    q->c.big_chunk_bytes = (long long)value << 20;
     return 0;
   }
   return fail("unknown queue tuning key '%s'", key);

@@ -138,33 +138,58 @@ __device__ __forceinline__ uint64_t pair_run_max(const cf *v, uint32_t lag0, int
   return a > b ? a : b;
 }
 
-__device__ __forceinline__ uint64_t xchg_max(uint64_t keep, uint64_t send, int m) {
-  const uint64_t got = __shfl_xor(send, m, 64);
-  return got > keep ? got : keep;
+// The transposing exchanges of the butterfly below, on 64-bit keys as two
+// 32-bit halves.  v_permlane32_swap on (lo, hi) gives lanes 0-31 the lo of
+// lane + 32 and lanes 32-63 the hi of lane - 32, with the own value in the
+// other result; v_permlane16_swap does the same between even and odd rows of
+// 16 lanes.  So the maximum of the two results is, in the lower half, lo
+// over the pair of lanes and, in the upper half, hi over it: one exchange
+// step with no selects and no LDS.
+__device__ __forceinline__ uint64_t max64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+__device__ __forceinline__ uint64_t join64(uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; }
+
+__device__ __forceinline__ uint64_t swap32_max(uint64_t lo, uint64_t hi) {
+  const auto a = __builtin_amdgcn_permlane32_swap((uint32_t)lo, (uint32_t)hi, false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap((uint32_t)(lo >> 32), (uint32_t)(hi >> 32), false, false);
+  return max64(join64(a[0], b[0]), join64(a[1], b[1]));
+}
+
+__device__ __forceinline__ uint64_t swap16_max(uint64_t lo, uint64_t hi) {
+  const auto a = __builtin_amdgcn_permlane16_swap((uint32_t)lo, (uint32_t)hi, false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap((uint32_t)(lo >> 32), (uint32_t)(hi >> 32), false, false);
+  return max64(join64(a[0], b[0]), join64(a[1], b[1]));
+}
+
+// The key of another lane of the same row of 16 through DPP.
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xf, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xf, 0xf, false);
+  return join64(lo, hi);
 }
 
 // R = 8 runs of two u each, streamed so that at most four partial maxima
 // are live at once.  Lane lags are lag0 + u*STEP; lane 0's lag0 is wave_lag0,
 // every lane's lag0 lies in [wave_lag0, wave_lag0 + STEP), and bins are 2 STEP
 // wide starting at lag 0, so run rho of every lane lies in bin
-// (wave_lag0 + 2 rho STEP) >> bshift.
+// (wave_lag0 + 2 rho STEP) >> bshift.  Lane pairs: l and l ^ 32, l and l ^ 16,
+// then l and its mirror 15 - l within the row (the other half of the row, as
+// l ^ 8 would be), then three maxima within groups of 8 lanes.
 template <int STEP>
 __device__ __forceinline__ void runs8_max_to_bins(const cf *v, uint32_t lag0, uint32_t wave_lag0, int lane,
                                                   float seed, int bshift, uint64_t *bins) {
   const bool up1 = (lane & 32) != 0, up2 = (lane & 16) != 0, up3 = (lane & 8) != 0;
   uint64_t q[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const uint64_t lo = pair_run_max<STEP>(v, lag0, i, seed);
-    const uint64_t hi = pair_run_max<STEP>(v, lag0, i + 4, seed);
-    q[i] = xchg_max(up1 ? hi : lo, up1 ? lo : hi, 32);
-  }
-  const uint64_t r0 = xchg_max(up2 ? q[2] : q[0], up2 ? q[0] : q[2], 16);
-  const uint64_t r1 = xchg_max(up2 ? q[3] : q[1], up2 ? q[1] : q[3], 16);
-  uint64_t w = xchg_max(up3 ? r1 : r0, up3 ? r0 : r1, 8);
-  w = xchg_max(w, w, 4);
-  w = xchg_max(w, w, 2);
-  w = xchg_max(w, w, 1);
+  for (int i = 0; i < 4; i++)
+    q[i] = swap32_max(pair_run_max<STEP>(v, lag0, i, seed), pair_run_max<STEP>(v, lag0, i + 4, seed));
+  const uint64_t r0 = swap16_max(q[0], q[2]);
+  const uint64_t r1 = swap16_max(q[1], q[3]);
+  uint64_t w = max64(up3 ? r1 : r0, dpp64<0x140>(up3 ? r0 : r1));  // row_mirror
+  w = max64(w, dpp64<0x141>(w));                                   // row_half_mirror
+  w = max64(w, dpp64<0x4E>(w));                                    // quad_perm(2, 3, 0, 1)
+  w = max64(w, dpp64<0xB1>(w));                                    // quad_perm(1, 0, 3, 2)
   const int rho = (up1 ? 4 : 0) + (up2 ? 2 : 0) + (up3 ? 1 : 0);
   if ((lane & 7) == 0 && w) {
     const uint32_t b = (wave_lag0 + (uint32_t)(rho * 2 * STEP)) >> bshift;

@@ -113,15 +113,6 @@ __device__ __forceinline__ uint64_t make_key(cf z, uint32_t k, float seed) {
   return m2 > seed ? ((uint64_t)__float_as_uint(m2) << 32) | (uint64_t)(~k) : 0ull;
 }
 
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint64_t o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
-  return v;
-}
-
 // Per-bin maxima of a wave's keys when the lag window is the whole transform
 // and bins are exactly 2 TPT lags wide: thread t's lags t + u*TPT, u < 16,
 // form 8 runs of two consecutive u, and each run lies in one bin for all 64
@@ -167,6 +158,18 @@ __device__ __forceinline__ uint64_t dpp64(uint64_t v) {
   const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xf, 0xf, false);
   const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xf, 0xf, false);
   return join64(lo, hi);
+}
+
+// Max of a 64-bit key over the wave, every lane receiving it: DPP within
+// rows of 16 lanes, then the row and half-wave swaps (call only with the whole
+// wave active).
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+  v = max64(v, dpp64<0xB1>(v));   // quad_perm(1, 0, 3, 2)
+  v = max64(v, dpp64<0x4E>(v));   // quad_perm(2, 3, 0, 1)
+  v = max64(v, dpp64<0x141>(v));  // row_half_mirror
+  v = max64(v, dpp64<0x140>(v));  // row_mirror
+  v = swap16_max(v, v);
+  return swap32_max(v, v);
 }
 
 // R = 8 runs of two u each, streamed so that at most four partial maxima

@@ -137,7 +137,9 @@ __device__ __forceinline__ uint64_t pair_run_max(const cf *v, uint32_t lag0, int
 // other result; v_permlane16_swap does the same between even and odd rows of
 // 16 lanes.  So the maximum of the two results is, in the lower half, lo
 // over the pair of lanes and, in the upper half, hi over it: one exchange
-// step with no selects and no LDS.
+// step with no selects and no LDS.  Like every cross-lane step here, these
+// read other lanes' registers whether or not those lanes are active, so they
+// are called only where the whole wave is (wave-uniform control flow).
 __device__ __forceinline__ uint64_t max64(uint64_t a, uint64_t b) { return a > b ? a : b; }
 
 __device__ __forceinline__ uint64_t join64(uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; }

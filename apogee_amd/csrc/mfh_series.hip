@@ -223,9 +223,18 @@ long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int s
   }
   std::vector<long long> st(nblocks);
   for (int b = 0; b < nblocks; b++) st[b] = start[b] < nseries ? (long long)start[b] : (long long)nseries;
+  // `st` must outlive its copy: the call ends with a stream synchronisation,
+  // and an early return waits for the stream (this guard, declared after
+  // `st`, is destroyed before it)
+  struct WaitOnReturn {
+    hipStream_t s;
+    bool armed = true;
+    ~WaitOnReturn() {
+      if (armed) hipStreamSynchronize(s);
+    }
+  } wait_on_return{c->stream};
   if (!hip_ok(hipMemcpyAsync(dstart, st.data(), nblocks * sizeof(long long), hipMemcpyHostToDevice, c->stream),
-              "copy starts") ||
-      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+              "copy starts"))
     return -1;
   if (mfh::launch_series_forward(c, dser, nseries, dstart, nblocks, spec) < 0) return -1;
   long long refined_total = 0;
@@ -268,6 +277,7 @@ long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int s
       !hip_ok(hipMemcpyAsync(val_c64, dval, cells * sizeof(cf), hipMemcpyDeviceToHost, c->stream), "copy val") ||
       !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
     return -1;
+  wait_on_return.armed = false;
   c->prof_collect();
   if (n_refined) *n_refined = refined_total;
   long total = 0;

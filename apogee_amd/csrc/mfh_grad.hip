@@ -126,7 +126,9 @@ __global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ o
   }
 }
 
-// z at stored lags: one wave per (pair, bin); FP32 products, FP64 sums.
+// z at stored lags: one wave per (pair, bin), FP32 throughout (FP64 on the
+// device is kept to table generation and band-fraction sums): each lane sums
+// its n/64 terms, then a fixed-order butterfly adds the lanes.
 __global__ void __launch_bounds__(256) peak_value_kernel(const cf *__restrict__ D, long long dstride,
                                                          const cf *__restrict__ T, long long tstride, int nt,
                                                          int nb, long long npeaks, const int64_t *__restrict__ idx,
@@ -139,7 +141,7 @@ __global__ void __launch_bounds__(256) peak_value_kernel(const cf *__restrict__ 
   const long long k = idx[p];
   const long long pair = p / nb;
   const int d = (int)(pair / nt), t = (int)(pair % nt);
-  double sx = 0.0, sy = 0.0;
+  float sx = 0.f, sy = 0.f;
   if (k >= 0) {
     const uint32_t mask = (uint32_t)n - 1u;
     const cf *dp = D + (long long)d * dstride;
@@ -155,7 +157,7 @@ __global__ void __launch_bounds__(256) peak_value_kernel(const cf *__restrict__ 
     sx += __shfl_xor(sx, off, 64);
     sy += __shfl_xor(sy, off, 64);
   }
-  if (lane == 0) out[p] = k >= 0 ? cf{(float)sx, (float)sy} : cf{0.f, 0.f};
+  if (lane == 0) out[p] = k >= 0 ? cf{sx, sy} : cf{0.f, 0.f};
 }
 
 }  // namespace

@@ -157,6 +157,50 @@ def test_ragged_bins_and_windows(n, ws, we, binsize, gpu_device):
                        context="%s" % ((n, ws, we, binsize),))
 
 
+def fuzz_cases(seed, n, count, with_window):
+    """Seeded (binsize, ws, we): binsize log-uniform in [1, n], so both many
+    small bins and one large bin occur; windows anywhere in [0, n)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        binsize = max(1, min(n, int(round(float(n) ** rng.uniform(0.0, 1.0)))))
+        if with_window:
+            ws = int(rng.integers(0, n - 1))
+            we = int(rng.integers(ws + 1, n + 1))
+        else:
+            ws, we = 0, n
+        out.append((binsize, ws, we))
+    return out
+
+
+@pytest.mark.parametrize("n", [1024, 2048, 4096, 65536])
+@pytest.mark.parametrize("with_window,seed", [(False, 20260923), (True, 8675309)])
+def test_fuzz_binsizes_and_windows(n, with_window, seed, gpu_device):
+    """Random bins and windows, as the reference's
+    tests/test_bin_window_matrix.py:197-231 fuzz, on the one-pass kernels and
+    (n = 65536) the four-step path."""
+    rng = np.random.default_rng(seed + n)
+    D, T = cnoise(rng, (1, n)), cnoise(rng, (2, n))
+    for binsize, ws, we in fuzz_cases(seed, n, 25, with_window):
+        pk = run(D, T, binsize, 1.0, (ws, we))
+        assert_peaks_match(D, T, pk["index"], pk["value"], binsize, 1.0, (ws, we),
+                           context="n=%d binsize=%d window=(%d,%d)" % (n, binsize, ws, we))
+
+
+@pytest.mark.parametrize("n", [1024, 16384])
+def test_fuzz_with_a_threshold_that_bites(n, gpu_device):
+    """Random bins and windows under a threshold at the 99th percentile of
+    |z|, so most bins are dismissed (the reference's
+    tests/test_bin_window_matrix.py:234-251)."""
+    rng = np.random.default_rng(3 + n)
+    D, T = cnoise(rng, (1, n)), cnoise(rng, (2, n))
+    thr = float(np.percentile(np.abs(O.correlate(D[0], T[0])), 99.0))
+    for binsize, ws, we in fuzz_cases(4242, n, 20, True):
+        pk = run(D, T, binsize, thr, (ws, we))
+        assert_peaks_match(D, T, pk["index"], pk["value"], binsize, thr, (ws, we),
+                           context="n=%d binsize=%d window=(%d,%d) thr=%.3f" % (n, binsize, ws, we, thr))
+
+
 @pytest.mark.parametrize("amp", [1e-12, 1e-6, 1.0, 1e6, 1e12])
 def test_amplitude_scale_moves_nothing(amp, gpu_device):
     n, lag = 2048, 733
@@ -207,6 +251,67 @@ def test_all_ties_report_the_lowest_lag(gpu_device):
             assert abs(pk["value"][0, 0, b] - z[i]) < 1e-5     # value belongs to its lag
         m2 = np.abs(pk["value"][0, 0]) ** 2
         assert m2.min() >= 1 - 1e-5
+
+
+@pytest.mark.parametrize("n", [1024, 4096, 16384, 32768, 65536])
+@pytest.mark.parametrize("binsize", [None, 257, 1])
+def test_tied_peaks_keep_index_and_value_together(n, binsize, gpu_device):
+    """The reference's tests/test_gpu_peak_ties.py:8-29: a delta at n/4 in data
+    and templates gives z[m] = i**m, |z| = 1 at every lag.  Whichever lag a bin
+    reports, it lies in the window and the bin, and its value is that lag's
+    z; a threshold of 2 empties every bin.  Repeated runs of one filter."""
+    rows = 32 if n == 16384 and binsize is None else 4
+    bank = np.zeros((rows, n), np.complex64)
+    bank[:, n // 4] = 1
+    f = mf.MatchedFilter(n, rows, rows, device="gpu")
+    f.set_data(bank)
+    f.set_templates(bank)
+    phases = np.array([1, 1j, -1, -1j], np.complex64)
+    for _ in range(12 if rows == 32 else 2):
+        out = f.run(binsize=binsize, window=(7, n - 3))
+        index = out["index"]
+        assert ((index >= 7) & (index < n - 3)).all()
+        np.testing.assert_allclose(out["value"], phases[index % 4], atol=2e-5)
+        if binsize is not None:
+            np.testing.assert_array_equal((index - 7) // binsize,
+                                          np.broadcast_to(np.arange(index.shape[-1]), index.shape))
+    empty = f.run(binsize=binsize, threshold=2.0, window=(7, n - 3))
+    np.testing.assert_array_equal(empty["index"], -1)
+    np.testing.assert_array_equal(empty["value"], 0)
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 4096, 16384, 65536])
+def test_equal_maxima_everywhere_report_each_bins_first_lag(n, gpu_device):
+    """The reference's tests/test_cpu_single_peak.py:7-34 contract, on the
+    device: a DC-only product makes z[m] = D[0] exactly at every lag (the
+    transform of a delta at frequency 0 only ever multiplies it by 1), so every
+    bin ties over all its lags and reports its first lag, unless the strict
+    threshold equals |D[0]| = 5.  Sub-ranges of templates, unaligned windows,
+    a one-sample window, binsizes above the window and the counts output."""
+    f = mf.MatchedFilter(n, 3, 37, device="gpu")
+    data = np.zeros((3, n), np.complex64)
+    data[:, 0] = [3 + 4j, -5j, 0]
+    templates = np.zeros((37, n), np.complex64)
+    templates[:, 0] = 1
+    f.set_templates(templates)
+    f.set_data(data)
+    for start, count in ((0, 1), (0, 37), (1, 33)):
+        for lo, hi in ((0, n), (7, n - 9), (11, 12)):
+            for binsize in (hi - lo, 17, n + 3):
+                for threshold in (0.0, 4.0, 5.0, 6.0):
+                    out, counts = f.run(templates=(start, count), window=(lo, hi),
+                                        binsize=binsize, threshold=threshold, counts=True)
+                    nbins = (hi - lo + binsize - 1) // binsize
+                    want_i = np.full((3, count, nbins), -1)
+                    want_v = np.zeros((3, count, nbins), np.complex64)
+                    if threshold < 5:
+                        want_i[:2] = lo + np.arange(nbins) * binsize
+                        want_v[:2] = data[:2, :1, None]
+                    ctx = "templates %s window %s binsize %d threshold %g" % (
+                        (start, count), (lo, hi), binsize, threshold)
+                    np.testing.assert_array_equal(out["index"], want_i, err_msg=ctx)
+                    np.testing.assert_array_equal(out["value"], want_v, err_msg=ctx)
+                    np.testing.assert_array_equal(counts, (want_i >= 0).sum(axis=-1), err_msg=ctx)
 
 
 def test_threshold_is_strict_on_the_fp32_key(gpu_device):

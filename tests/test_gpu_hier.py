@@ -161,6 +161,30 @@ def test_explicit_threshold_zero_refines_everything(gpu_device):
     assert rates[3.5] >= rates[4.5]
 
 
+@pytest.mark.parametrize("band,prec", [(64, "fp32"), (128, "fp32"), (256, "fp32"), (512, "fp32"),
+                                       (1024, "fp32"), (2048, "fp32"), (512, "fp16"), (512, "bf16")])
+def test_ties_through_coarse_compaction_and_refinement(band, prec, gpu_device):
+    """The reference's tests/test_gpu_peak_ties.py:32-51: a delta at band/4,
+    inside the coarse band, ties |z| = 1 at every coarse and every full lag.
+    Every pair clears a coarse gate of 0.5, is compacted and refined, and the
+    reported value is the full correlation at the reported lag."""
+    n = 4096
+    bank = np.zeros((4, n), np.complex64)
+    bank[:, band // 4] = 1
+    f = mf.HierarchicalFilter(n, 4, 4, band=band, device="gpu")
+    f.set_coarse_threshold(0.5)
+    if prec != "fp32":
+        f.set_coarse_precision(prec)
+    f.set_data(bank)
+    f.set_templates(bank)
+    for _ in range(3):
+        out = f.run()
+        assert (out["index"] >= 0).all()
+        expected = np.exp(2j * np.pi * (band // 4) * out["index"] / n)
+        np.testing.assert_allclose(out["value"], expected, atol=3e-5)
+    assert f.stats == (48, 48)
+
+
 def test_omission_rate_meets_the_budget(gpu_device):
     """Measured false dismissal of SNR-5.5 injections against fd = 1e-2."""
     n, trials, snr, fd = 4096, 4096, 5.5, 1e-2

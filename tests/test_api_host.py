@@ -244,6 +244,43 @@ def test_run_series_contract(fake):
         f.run()                                                    # series consumed the slots
 
 
+@pytest.mark.parametrize("hier", [False, True])
+@pytest.mark.parametrize("case", ["no_blocks", "series_2d", "starts_2d", "window_2d", "negative_start",
+                                  "huge_start", "empty_templates", "past_templates", "zero_binsize",
+                                  "negative_binsize"])
+def test_invalid_series_layout_fails_before_the_device(fake, case, hier):
+    """The reference's tests/test_series_lifecycle.py:83-117: every malformed
+    series layout raises ValueError in Python and never reaches the device."""
+    n = 1024
+    f = mf.HierarchicalFilter(n, 1, 1, band=128) if hier else mf.MatchedFilter(n, 1, 1)
+    f.set_templates(np.ones((1, n), np.complex64))
+    if hier:
+        f.set_reference(np.ones(n, np.float32))
+    kw = dict(series=np.ones(n, np.complex64), starts=np.array([0]),
+              win_start=np.array([0]), win_end=np.array([n]), binsize=n)
+    if case == "no_blocks":
+        kw.update(starts=[], win_start=[], win_end=[])
+    elif case == "series_2d":
+        kw["series"] = kw["series"].reshape(2, n // 2)
+    elif case == "starts_2d":
+        kw["starts"] = np.array([[0]])
+    elif case == "window_2d":
+        kw["win_end"] = np.array([[n]])
+    elif case == "negative_start":
+        kw["starts"] = np.array([-1], dtype=np.int64)
+    elif case == "huge_start":
+        kw["starts"] = np.array([np.iinfo(np.uintp).max], dtype=np.uintp)
+    elif case == "empty_templates":
+        kw["templates"] = (0, 0)
+    elif case == "past_templates":
+        kw["templates"] = (1, 2)
+    else:
+        kw["binsize"] = 0 if case == "zero_binsize" else -1
+    with pytest.raises(ValueError):
+        f.run_series(**kw)
+    assert not [c for c in FakeContext.calls if c[0] == "series"]
+
+
 def test_autotune_table_lookup(tmp_path, monkeypatch):
     """Nearest entry of the same operation and length, in log (ndata, ntemplates)."""
     import json

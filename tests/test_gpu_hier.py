@@ -161,6 +161,84 @@ def test_explicit_threshold_zero_refines_everything(gpu_device):
     assert rates[3.5] >= rates[4.5]
 
 
+@pytest.mark.parametrize("band", [128, 256, 512])
+def test_gate_holds_under_scaling_and_bank_partition(band, gpu_device):
+    """The reference's tests/test_calibration_invariants.py:19-69: with a
+    coarse gate in a wide gap of the FP64 coarse maxima (so rounding cannot
+    move a decision), the hierarchical filter reports exactly the flat peaks
+    of the admitted pairs, whatever the template order, the partition of the
+    bank over filters, a reciprocal scaling of data and templates or the
+    amplitude of the reference; a changed reference rescales the coarse
+    templates, moving the decisions to the new band fraction."""
+    n, nd, nt = 1024, 12, 4
+    rng = np.random.default_rng(810 + band)
+    data = (rng.normal(size=(nd, n)) + 1j * rng.normal(size=(nd, n))).astype(np.complex64)
+    templates = (rng.normal(size=(nt, n)) + 1j * rng.normal(size=(nt, n))).astype(np.complex64)
+    templates *= np.exp(-np.arange(n)[None, :] / np.array([80, 160, 320, 640])[:, None])
+    templates /= np.linalg.norm(templates, axis=1)[:, None]
+    reference = np.exp(-np.arange(n) / 240).astype(np.float32)
+    fraction = reference[:band].astype(float).sum() / reference.astype(float).sum()
+    product = data[:, None, :band].astype(np.complex128) * np.conj(templates[None, :, :band])
+    coarse = np.abs(np.fft.ifft(product, axis=-1) * band).max(axis=-1) / np.sqrt(fraction)
+    ordered = np.sort(coarse.ravel())
+    lo, hi = len(ordered) // 3, 2 * len(ordered) // 3
+    gap = lo + int(np.argmax(np.diff(ordered)[lo:hi]))
+    threshold = (ordered[gap] + ordered[gap + 1]) / 2
+    assert ordered[gap + 1] - ordered[gap] > 0.01 * threshold
+    admitted = coarse >= threshold
+    assert admitted.any() and (~admitted).any()
+    flat = mf.MatchedFilter(n, nd, nt, device="gpu")
+    flat.set_data(data)
+    flat.set_templates(templates)
+    expected = flat.run().copy()
+    expected["index"][~admitted] = -1
+    expected["value"][~admitted] = 0
+    for scale, refscale, chunks in [(1.0, 1.0, [(0, 4)]), (4.0, 8.0, [(0, 1), (1, 4)])]:
+        order = np.arange(nt)[::-1]
+        for begin, end in chunks:
+            cols = order[begin:end]
+            f = mf.HierarchicalFilter(n, nd, len(cols), band=band, device="gpu")
+            f.set_coarse_threshold(float(threshold))
+            f.set_reference(reference * refscale)
+            f.set_templates(templates[cols] / scale)
+            f.set_data(data * scale)
+            got = f.run().copy()
+            np.testing.assert_array_equal(got["index"], expected["index"][:, cols])
+            np.testing.assert_array_equal(got["value"], expected["value"][:, cols])
+            assert f.stats == (nd * len(cols), int(admitted[:, cols].sum()))
+            newref = reference.copy()
+            newref[:band] *= 0.5
+            f.set_reference(newref)
+            newfraction = newref[:band].astype(float).sum() / newref.astype(float).sum()
+            newmask = coarse[:, cols] * np.sqrt(fraction / newfraction) >= threshold
+            changed = f.run()
+            np.testing.assert_array_equal(changed["index"] >= 0, np.broadcast_to(newmask[..., None],
+                                                                                  changed["index"].shape))
+
+
+def test_reference_update_recalibrates_a_reused_filter(gpu_device):
+    """The reference's tests/test_series_lifecycle.py:120-142: a flat spectrum
+    of which only the band survives in the data gives a coarse maximum of
+    band / (n sqrt(f)): 0.5 at band fraction 1/4 and 0.293 at 8/11.  With the
+    gate between them, each new reference must reach the executed gate."""
+    n, band = 2048, 512
+    f = mf.HierarchicalFilter(n, band=band, snr=5.5, fd=1e-2, device="gpu")
+    f.set_coarse_threshold(0.35)
+    h = np.full((1, n), 1.0 / np.sqrt(n), np.complex64)
+    d = h.copy()
+    d[:, band:] = 0
+    f.set_templates(h)
+    f.set_data(d)
+    for low_power in (1.0, 8.0, 1.0):
+        ref = np.ones(n, np.float32)
+        ref[:band] = low_power
+        f.set_reference(ref)
+        peaks = f.run(binsize=n)
+        assert (peaks["index"] >= 0).all() == (low_power == 1.0)
+        fraction = ref[:band].astype(float).sum() / ref.astype(float).sum()
+        assert f.coarse_statistic("fp32")[0, 0] == pytest.approx(band / (n * np.sqrt(fraction)), rel=1e-6)
+
+
 @pytest.mark.parametrize("band,prec", [(64, "fp32"), (128, "fp32"), (256, "fp32"), (512, "fp32"),
                                        (1024, "fp32"), (2048, "fp32"), (512, "fp16"), (512, "bf16")])
 def test_ties_through_coarse_compaction_and_refinement(band, prec, gpu_device):

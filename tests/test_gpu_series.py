@@ -140,6 +140,37 @@ def test_hierarchical_series_is_identical_or_absent(gpu_device):
     assert pairs == len(starts) * nt and trig < pairs
 
 
+@pytest.mark.parametrize("nblk", [1, 4, 6])
+def test_hierarchical_series_dismisses_pure_noise(nblk, gpu_device):
+    """The reference's tests/test_hier_series_scale.py:45-75: block spectra
+    carry the 1/n of the forward transform on the hierarchical route as on the
+    flat one, so on unit complex noise the flat peaks stay near 0.07 and no
+    pair clears a gate calibrated for SNR 5.5."""
+    n, nt = 4096, 8
+    rng = np.random.default_rng(3)
+    k = np.arange(1, n // 2)
+    power = np.zeros(n, np.float32)
+    power[1:n // 2] = k ** (-7 / 3.0) / ((0.015 * n / k) ** 4 + 1.0)
+    power /= power.sum()
+    H = np.stack([np.sqrt(power) * np.exp(2j * np.pi * rng.random(n)) for _ in range(nt)])
+    H = (H / np.linalg.norm(H, axis=1, keepdims=True)).astype(np.complex64)
+    step = n // 2
+    series = (rng.standard_normal(n + step * nblk) + 1j * rng.standard_normal(n + step * nblk)).astype(np.complex64)
+    starts = (np.arange(nblk) * step).astype(np.uintp)
+    ws = np.zeros(nblk, np.uintp)
+    we = np.full(nblk, n // 2, np.uintp)
+    flat = mf.MatchedFilter(n, nblk, nt, device="gpu")
+    flat.set_templates(H)
+    a = flat.run_series(series, starts, ws, we, binsize=n, threshold=0.0)
+    assert (a["index"] >= 0).all() and np.abs(a["value"]).max() < 0.2
+    hier = mf.HierarchicalFilter(n, nblk, nt, snr=5.5, fd=1e-2, device="gpu")
+    hier.set_reference(power)
+    hier.set_templates(H)
+    b = hier.run_series(series, starts, ws, we, binsize=n, threshold=0.0)
+    assert (b["index"] < 0).all(), "%d peaks reported on pure noise" % (b["index"] >= 0).sum()
+    assert hier.stats == (nblk * nt, 0)
+
+
 def test_device_resident_series(gpu_device):
     import torch
     n, nt = 1024, 2

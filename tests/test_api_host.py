@@ -159,6 +159,19 @@ def test_hierarchical_validation_and_config(fake):
     assert p.coarse_threshold == 2.5
 
 
+@pytest.mark.parametrize("band", [512, 1024])
+def test_taps_is_inert(fake, band):
+    """The reference's tests/test_taps_is_inert.py:15-28: taps is metadata;
+    the modelled gate does not move with it."""
+    gates = set()
+    for taps in (2, 4, 8):
+        h = mf.HierarchicalFilter(4096, 2, 2, snr=5.0, fd=1e-2, band=band, taps=taps)
+        h.set_reference(inspiral_power(4096))
+        assert h.config == (band, taps)
+        gates.add(h.coarse_threshold)
+    assert len(gates) == 1, gates
+
+
 def test_hierarchical_run_is_identical_or_absent_and_counts_triggers(fake):
     rng = np.random.default_rng(11)
     n, nd, nt = 1024, 4, 4

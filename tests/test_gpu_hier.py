@@ -263,14 +263,17 @@ def test_ties_through_coarse_compaction_and_refinement(band, prec, gpu_device):
     assert f.stats == (48, 48)
 
 
-def test_omission_rate_meets_the_budget(gpu_device):
-    """Measured false dismissal of SNR-5.5 injections against fd = 1e-2."""
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "bf16"])
+def test_omission_rate_meets_the_budget(prec, gpu_device):
+    """Measured false dismissal of SNR-5.5 injections against fd = 1e-2, for
+    the FP32 screen and the widened matrix-core screens (same trials)."""
     n, trials, snr, fd = 4096, 4096, 5.5, 1e-2
     rng = np.random.default_rng(13)
     power = inspiral_power(n)
     H = template_with_power(n, power)
     nd = 512
     flat, hier = build(n, nd, 1, power, snr=snr, fd=fd)
+    hier.set_coarse_precision(prec)
     flat.set_templates(H[None, :])
     hier.set_templates(H[None, :])
     ph = np.exp(2j * np.pi * np.arange(n) / n)
@@ -287,7 +290,7 @@ def test_omission_rate_meets_the_budget(gpu_device):
         omitted += int(((a >= 0) & (b < 0)).sum())
     assert detected > 1000
     rate = omitted / detected
-    print("fd budget %.3g: omitted %d of %d detections (%.4f)" % (fd, omitted, detected, rate))
+    print("%s fd budget %.3g: omitted %d of %d detections (%.4f)" % (prec, fd, omitted, detected, rate))
     assert rate <= 2 * fd, rate
 
 

@@ -117,12 +117,12 @@ int launch_len(Ctx *c, const FlatRun &r) {
   return 0;
 }
 
-// Rows r = 1, 2, 4, 8 of exp(+2 pi i r k / (16 ns)), k < ns, at [row][k],
+// Rows r = 1, 4 of exp(+2 pi i r k / (16 ns)), k < ns, at [row][k],
 // generated in FP64.
 __global__ void twiddle_pass_kernel(cf *out, int ns) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ns * 4) {
-    const int r = 1 << (i / ns), k = i % ns;
+  if (i < ns * kTwiddleRows) {
+    const int r = i < ns ? 1 : 4, k = i % ns;
     double s, c;
     sincospi(2.0 * (double)r * (double)k / (double)(16 * ns), &s, &c);
     out[i] = cf{(float)c, (float)s};
@@ -136,7 +136,7 @@ int fill_twiddles(Ctx *c, cf *tw) {
   int ns = L::NS0;
   for (int p = 0; p < S::NPASS16; p++, ns *= 16) {
     if (ns == 1) continue;
-    const int cnt = ns * 4;
+    const int cnt = ns * kTwiddleRows;
     hipLaunchKernelGGL(twiddle_pass_kernel, dim3((cnt + 255) / 256), dim3(256), 0, c->stream,
                        tw + L::offset(p), ns);
   }

@@ -171,11 +171,15 @@ def flat_peaks(D, T, binsize=None, threshold=0.0, win=None):
 
 
 def band_fraction(T, band, reference=None):
-    """In-band power fraction f, in FP64 (src/hmf.c:125-137, :147-152).
+    """In-band power fraction f, in FP64.
 
     With a reference profile: f = sum_{k<band} ref / sum ref, one value for the
-    whole bank.  Without: per template, sum_{k<band} |T|^2 / sum |T|^2 (0 for
-    an all-zero template).
+    whole bank (src/hmf.c:125-137; python/matchedfilter/__init__.py:1276-1278).
+    Without: per template, sum_{k<band} |T|^2 / sum |T|^2 (0 for an all-zero
+    template), kept in FP64 as the reference's GPU dispatch does
+    (python/matchedfilter/__init__.py:1290-1300); the reference's CPU engine
+    rounds this per-template fraction to float (src/hmf.c:147-152), a
+    difference of one FP32 rounding in the coarse scale.
     """
     if reference is not None:
         ref = np.asarray(reference, dtype=np.float64)

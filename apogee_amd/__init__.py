@@ -38,10 +38,6 @@ _LARGE_SIZES = frozenset((131072, 262144, 524288, 1048576))
 __all__ = ["MatchedFilter", "HierarchicalFilter", "RefineQueue", "PEAK_DTYPE", "devices",
            "Device", "UnsupportedSize", "__version__"]
 
-#: DLPack device types read as host memory: CPU, CUDA host, ROCm host.
-_DLPACK_HOST = {1, 3, 11}
-_DLPACK_ROCM = 10
-
 
 def devices():
     """Every HIP device this build can dispatch to."""
@@ -49,73 +45,148 @@ def devices():
     return _devices()
 
 
-def _from_any(a):
-    """numpy arrays pass through; DLPack host arrays are wrapped; ROCm
-    tensors pass through untouched for a device-to-device copy; any other
-    accelerator array is refused (python/matchedfilter/__init__.py:96-131)."""
+# ------------------------------------------------------------------ inputs
+#: DLPack device types whose memory the host can read directly (CPU, CUDA
+#: pinned host, ROCm pinned host).
+_HOST_KINDS = frozenset((1, 3, 11))
+
+
+def _dlpack_kind(a):
+    """DLPack device type of ``a`` (1 = host when it does not say)."""
+    probe = getattr(a, "__dlpack_device__", None)
+    if probe is None:
+        return 1
+    try:
+        return int(probe()[0])
+    except Exception:
+        return 1
+
+
+def _on_rocm(a):
+    """True for a ROCm-resident array this build reads through DLPack."""
+    return _dlpack_kind(a) == _hip.DLPACK_ROCM and hasattr(a, "__dlpack__")
+
+
+def _host_view(a):
+    """A host array as NumPy (DLPack host producers are wrapped without a
+    copy where NumPy can); arrays on any accelerator other than this build's
+    ROCm device raise TypeError, because a silent transfer to the host would
+    hide the cost the caller came to avoid."""
     if isinstance(a, np.ndarray):
         return a
-    if hasattr(a, "__dlpack_device__"):
-        try:
-            kind = int(a.__dlpack_device__()[0])
-        except Exception:
-            kind = 1
-        if kind == _DLPACK_ROCM and hasattr(a, "data_ptr"):
-            return a
-        if kind not in _DLPACK_HOST:
-            raise TypeError("array is on a non-host device (DLPack type %d) that "
-                            "this build cannot read; transfer it explicitly" % kind)
+    kind = _dlpack_kind(a)
+    if kind not in _HOST_KINDS:
+        raise TypeError("array is on a non-host device (DLPack type %d) that this build "
+                        "cannot read; transfer it explicitly" % kind)
+    if hasattr(a, "__dlpack__"):
         try:
             return np.from_dlpack(a)
-        except Exception:
+        except Exception:       # read-only or exotic producers: NumPy coerces below
             pass
-    return a
+    return np.asarray(a)
 
 
-def _is_device_tensor(a):
-    return hasattr(a, "data_ptr") and getattr(a, "is_cuda", False)
+def _c64_rows(a, shape, what):
+    """Host complex64 C-contiguous array of exactly ``shape``."""
+    out = np.ascontiguousarray(_host_view(a), dtype=np.complex64)
+    if len(shape) == 1:
+        if out.ndim != 1 or out.size != shape[0]:
+            raise ValueError("%s must be a 1-D complex array of %d samples, got shape %s"
+                             % (what, shape[0], out.shape))
+    elif out.shape != shape:
+        raise ValueError("expected shape %s, got %s" % (shape, out.shape))
+    return out
 
 
-def _device_c64(a, shape, what):
-    """Validate a ROCm tensor as complex64 of ``shape``; returns it contiguous."""
-    import torch
-    if a.dtype != torch.complex64:
-        raise ValueError("%s must be complex64, got %s" % (what, a.dtype))
-    if tuple(a.shape) != tuple(shape):
-        raise ValueError("expected shape %s, got %s" % (tuple(shape), tuple(a.shape)))
-    return a.contiguous()
+def _rocm_rows(ctx, a, shape, what):
+    """Borrow a ROCm producer's complex64 buffer of ``shape`` for ``ctx``.
+
+    A PyTorch tensor is made contiguous first; other producers must hand
+    over compact (row-major) complex64 memory on the context's device."""
+    if hasattr(a, "is_contiguous") and hasattr(a, "contiguous") and not a.is_contiguous():
+        a = a.contiguous()
+    view = ctx.borrow(a)
+    if not view.complex64:
+        raise ValueError("%s must be complex64" % what)
+    got = view.shape if len(shape) > 1 else (int(np.prod(view.shape)),)
+    if tuple(got) != tuple(shape):
+        raise ValueError("expected shape %s, got %s" % (shape, view.shape))
+    if not view.compact:
+        raise ValueError("%s must be C-contiguous device memory" % what)
+    if view.device != ctx.index:
+        raise ValueError("%s is on device %d, the filter on %d" % (what, view.device, ctx.index))
+    return view
 
 
-def _as_c64(a, n, what):
-    a = np.ascontiguousarray(_from_any(a), dtype=np.complex64)
-    if a.ndim != 1 or a.size != n:
-        raise ValueError("%s must be a 1-D complex array of %d samples, got shape %s"
-                         % (what, n, a.shape))
-    return a
+# ----------------------------------------------------------------- results
+def _records(idx, val):
+    """(index, value) as one PEAK_DTYPE array."""
+    out = np.empty(idx.shape, PEAK_DTYPE)
+    out["index"] = idx
+    out["value"] = val
+    return out
 
 
-def _format_result(idx, val, *, raw=False, counts=None, order=None):
-    """Assemble PEAK_DTYPE records or return raw arrays, optionally restoring
-    caller order (python/matchedfilter/__init__.py:155-173)."""
-    if counts is True:
-        counts = (idx >= 0).sum(axis=2).astype(np.int32)
-    if raw:
-        if order is None:
-            result = (idx, val)
+def _result(idx, val, raw, counts):
+    """The public return value: records (or the raw pair), plus the per-pair
+    crossing counts when they were asked for."""
+    body = (idx, val) if raw else _records(idx, val)
+    return body if counts is None else (body, counts)
+
+
+# ----------------------------------------------------- request resolution
+class _RowBook:
+    """Which rows of a bank have been set since the filter was made."""
+
+    def __init__(self, rows):
+        self._have = np.zeros(rows, dtype=bool)
+        self._full = False
+
+    def record(self, row):
+        if row is None:
+            self._full = True
         else:
-            ri = np.empty(idx.shape, np.int64)
-            rv = np.empty(val.shape, np.complex64)
-            ri[order], rv[order] = idx, val
-            result = (ri, rv)
-    else:
-        result = np.empty(idx.shape, PEAK_DTYPE)
-        if order is None:
-            result["index"], result["value"] = idx, val
-        else:
-            result["index"][order], result["value"][order] = idx, val
-    if counts is not None and counts is not False:
-        return result, counts
-    return result
+            self._have[row] = True
+            self._full = bool(self._have.all())
+
+    def forget(self):
+        self._have[:] = False
+        self._full = False
+
+    def covers(self, first, count):
+        return self._full or bool(self._have[first:first + count].all())
+
+
+def _clamp_window(n, window):
+    """Lag window (start, end) clamped to [0, n]; empty raises ValueError."""
+    if window is None:
+        return 0, n
+    lo = min(max(int(window[0]), 0), n)
+    hi = min(max(int(window[1]), 0), n)
+    if hi <= lo:
+        raise ValueError("empty window (%d, %d)" % (lo, hi))
+    return lo, hi
+
+
+def _bin_width(binsize, n):
+    width = n if binsize is None else int(binsize)
+    if width < 1:
+        raise ValueError("binsize must be >= 1")
+    return width
+
+
+def _row_span(span, total):
+    """(first, count) of a sub-range argument; None is the whole bank."""
+    return (0, total) if span is None else (int(span[0]), int(span[1]))
+
+
+def _row_index(index, rows):
+    if index is None:
+        return None
+    row = int(index)
+    if not 0 <= row < rows:
+        raise IndexError("index %d out of range" % row)
+    return row
 
 
 class MatchedFilter:
@@ -151,107 +222,83 @@ class MatchedFilter:
         autotune.apply(self._gpu, "flat", self.n, self.ndata, self.ntemplates)
 
     def _init_state(self):
-        self._dataset = False
-        self._data_ready = set()
-        self._template_ready = set()
+        self._books = {"data": _RowBook(self.ndata), "template": _RowBook(self.ntemplates)}
+        self._has_data = False
         self._series_batch_bytes = 64 * 1024 * 1024
         self._cache_bytes = 512 * 1024 * 1024
 
     # ------------------------------------------------------------- ingest
-    def _input_index(self, index, size):
-        if index is None:
-            return None
-        index = int(index)
-        if index < 0 or index >= size:
-            raise IndexError("index %d out of range" % index)
-        return index
-
-    def _mark_ready(self, kind, index):
-        attr = "_" + kind + "_ready"
-        ready = getattr(self, attr)
-        if index is None:
-            setattr(self, attr, None)
-        elif ready is not None:
-            ready.add(index)
-            size = self.ndata if kind == "data" else self.ntemplates
-            if len(ready) == size:
-                setattr(self, attr, None)
-
-    def _missing(self, kind, start, count):
-        ready = getattr(self, "_" + kind + "_ready")
-        return ready is not None and any(i not in ready for i in range(start, start + count))
-
-    def _require_templates(self, start, count):
-        if self._missing("template", start, count):
-            raise ValueError("no templates for requested rows: call set_templates() first")
-
-    def _ingest(self, kind, spectra, index):
+    def _ingest(self, kind, spectra, row):
+        """Put one row (``row`` an int) or the whole bank (``row`` None) on
+        the device.  Host arrays are copied.  ROCm arrays are taken through
+        DLPack, ordered after the producer's pending work on the device: a
+        whole bank is read in place (bound, not copied), a single row is
+        copied device to device."""
         rows = self.ndata if kind == "data" else self.ntemplates
-        a = _from_any(spectra)
-        if index is None:
-            if _is_device_tensor(a):
-                a = _device_c64(a, (rows, self.n), "spectra")
+        shape = (rows, self.n) if row is None else (self.n,)
+        what = "spectra" if row is None else "spectrum"
+        if _on_rocm(spectra):
+            view = _rocm_rows(self._gpu, spectra, shape, what)
+            if row is None:
+                self._gpu.bind_rows(kind, view)
             else:
-                a = np.ascontiguousarray(a, dtype=np.complex64)
-                if a.shape != (rows, self.n):
-                    raise ValueError("expected shape (%d, %d), got %s" % (rows, self.n, a.shape))
-            self._gpu.set_rows(kind, 0, a)
+                self._gpu.copy_rows(kind, row, view)
+            a = spectra
         else:
-            if _is_device_tensor(a):
-                a = _device_c64(a.reshape(-1), (self.n,), "spectrum").reshape(1, self.n)
-            else:
-                a = _as_c64(a, self.n, "spectrum")[None, :]
-            self._gpu.set_rows(kind, index, a)
-        self._on_ingest(kind, index, a)
+            a = _c64_rows(spectra, shape, what)
+            self._gpu.set_rows(kind, 0 if row is None else row, a.reshape(-1, self.n))
+        self._on_ingest(kind, row, a)
 
     def _on_ingest(self, kind, index, a):
         """Hook for subclasses that derive state from the spectra."""
 
     def set_data(self, spectra, index=None):
-        """Set one data spectrum (with ``index``) or all, from (ndata, n)."""
-        index = self._input_index(index, self.ndata)
-        self._ingest("data", spectra, index)
-        self._dataset = True
-        self._mark_ready("data", index)
+        """Set one data spectrum (with ``index``) or all, from (ndata, n).
+
+        Host arrays are copied.  A ROCm array (any DLPack producer, e.g. a
+        PyTorch tensor) of the whole bank is read in place by every later
+        run: writing to it afterwards requires calling ``set_data`` again,
+        which orders the next run after the write."""
+        row = _row_index(index, self.ndata)
+        self._ingest("data", spectra, row)
+        self._has_data = True
+        self._books["data"].record(row)
 
     def set_templates(self, spectra, index=None):
-        """Set one template spectrum (with ``index``) or all, from (ntemplates, n)."""
-        index = self._input_index(index, self.ntemplates)
-        self._ingest("template", spectra, index)
-        self._mark_ready("template", index)
+        """Set one template spectrum (with ``index``) or all, from
+        (ntemplates, n); ROCm arrays as for :meth:`set_data`."""
+        row = _row_index(index, self.ntemplates)
+        self._ingest("template", spectra, row)
+        self._books["template"].record(row)
 
     # ---------------------------------------------------------------- run
     def _window(self, window):
-        if window is None:
-            return 0, self.n
-        start, end = int(window[0]), int(window[1])
-        start = max(0, min(start, self.n))
-        end = max(0, min(end, self.n))
-        if start >= end:
-            raise ValueError("empty window (%d, %d)" % (start, end))
-        return start, end
+        return _clamp_window(self.n, window)
 
     def nbins(self, binsize, window=None):
-        binsize = int(binsize)
-        if binsize < 1:
-            raise ValueError("binsize must be >= 1")
-        start, end = self._window(window)
-        return -(-(end - start) // binsize)
+        width = _bin_width(binsize, self.n)
+        lo, hi = self._window(window)
+        return (hi - lo + width - 1) // width
+
+    def _require_templates(self, first, count):
+        if not self._books["template"].covers(first, count):
+            raise ValueError("no templates for requested rows: call set_templates() first")
 
     def _check_run(self, binsize, window, data, templates):
-        binsize = self.n if binsize is None else int(binsize)
-        if binsize < 1:
-            raise ValueError("binsize must be >= 1")
-        start, end = self._window(window)
-        d0, nd = (0, self.ndata) if data is None else (int(data[0]), int(data[1]))
-        t0, nt = (0, self.ntemplates) if templates is None else (int(templates[0]), int(templates[1]))
-        if nd < 1 or nt < 1 or d0 < 0 or t0 < 0 \
-                or d0 + nd > self.ndata or t0 + nt > self.ntemplates:
+        """Resolve a run's arguments to (binsize, start, end, d0, nd, t0, nt),
+        with the reference's errors and their order."""
+        width = _bin_width(binsize, self.n)
+        lo, hi = self._window(window)
+        d0, nd = _row_span(data, self.ndata)
+        t0, nt = _row_span(templates, self.ntemplates)
+        inside = (min(nd, nt) >= 1 and min(d0, t0) >= 0
+                  and d0 + nd <= self.ndata and t0 + nt <= self.ntemplates)
+        if not inside:
             raise ValueError("data/templates sub-range out of bounds")
-        if not self._dataset or self._missing("data", d0, nd):
+        if not (self._has_data and self._books["data"].covers(d0, nd)):
             raise ValueError("no data: call set_data() before run()")
         self._require_templates(t0, nt)
-        return binsize, start, end, d0, nd, t0, nt
+        return width, lo, hi, d0, nd, t0, nt
 
     def run(self, binsize=None, threshold=0.0, window=None,
             data=None, templates=None, counts=False, raw=False):
@@ -264,9 +311,10 @@ class MatchedFilter:
         (identical to the matching slice of a full run); ``window=(start,
         end)`` restricts the lags searched.
         """
-        binsize, start, end, d0, nd, t0, nt = self._check_run(binsize, window, data, templates)
-        idx, val, cnt = self._dispatch(d0, nd, t0, nt, binsize, threshold, start, end, counts)
-        return _format_result(idx, val, raw=raw, counts=cnt if counts else None)
+        req = self._check_run(binsize, window, data, templates)
+        width, lo, hi, d0, nd, t0, nt = req
+        idx, val, cnt = self._dispatch(d0, nd, t0, nt, width, threshold, lo, hi, counts)
+        return _result(idx, val, raw, cnt if counts else None)
 
     def _dispatch(self, d0, nd, t0, nt, binsize, threshold, start, end, counts):
         return self._gpu.peaks(d0, nd, t0, nt, binsize, float(threshold), start, end,
@@ -274,39 +322,39 @@ class MatchedFilter:
 
     # ------------------------------------------------------------ series
     def _series_layout(self, series, starts, win_start, win_end, binsize, templates):
-        """Validate the run_series contract (python/matchedfilter/__init__.py:517-543)."""
-        ser = _from_any(series)
-        if _is_device_tensor(ser):
+        """Check a run_series request (the reference's contract,
+        python/matchedfilter/__init__.py:517-543) and build its layout."""
+        flat_err = "series, starts, win_start and win_end must be one-dimensional"
+        if _on_rocm(series):
             import torch
+            ser = torch.from_dlpack(series) if not isinstance(series, torch.Tensor) else series
             if ser.dim() != 1:
-                raise ValueError("series, starts, win_start and win_end must be one-dimensional")
+                raise ValueError(flat_err)
             ser = ser.to(torch.complex64).contiguous()
         else:
-            ser = np.ascontiguousarray(ser, dtype=np.complex64)
+            ser = np.ascontiguousarray(_host_view(series), dtype=np.complex64)
             if ser.ndim != 1:
-                raise ValueError("series, starts, win_start and win_end must be one-dimensional")
-        arrs = [np.asarray(_from_any(a)) for a in (starts, win_start, win_end)]
-        if any(a.ndim != 1 for a in arrs):
-            raise ValueError("series, starts, win_start and win_end must be one-dimensional")
-        if any(np.issubdtype(a.dtype, np.signedinteger) and (a < 0).any() for a in arrs):
-            raise ValueError("starts and windows must be nonnegative")
-        st, ws, we = (np.ascontiguousarray(a, dtype=np.uintp) for a in arrs)
-        if not (st.size == ws.size == we.size):
+                raise ValueError(flat_err)
+        offsets = [np.asarray(_host_view(a)) for a in (starts, win_start, win_end)]
+        if any(a.ndim != 1 for a in offsets):
+            raise ValueError(flat_err)
+        for a in offsets:
+            if np.issubdtype(a.dtype, np.signedinteger) and a.size and a.min() < 0:
+                raise ValueError("starts and windows must be nonnegative")
+        st, ws, we = (np.ascontiguousarray(a, dtype=np.uintp) for a in offsets)
+        if len({st.size, ws.size, we.size}) != 1:
             raise ValueError("starts, win_start and win_end must be the same length")
-        if st.size < 1:
+        if st.size == 0:
             raise ValueError("run_series needs at least one block")
-        if np.any(st > np.iinfo(np.intp).max - self.n):
+        if (st > np.iinfo(np.intp).max - self.n).any():
             raise ValueError("starts must be nonnegative and fit in the sample index range")
-        binsize = self.n if binsize is None else int(binsize)
-        if binsize < 1:
-            raise ValueError("binsize must be >= 1")
-        t0, nt = (0, self.ntemplates) if templates is None else (int(templates[0]), int(templates[1]))
+        width = _bin_width(binsize, self.n)
+        t0, nt = _row_span(templates, self.ntemplates)
         if nt < 1 or t0 < 0 or t0 + nt > self.ntemplates:
             raise ValueError("templates sub-range out of bounds")
         self._require_templates(t0, nt)
         from ._series import SeriesLayout
-        layout = SeriesLayout(self.n, st, ws, we, binsize)
-        return ser, layout, binsize, t0, nt
+        return ser, SeriesLayout(self.n, st, ws, we, width), width, t0, nt
 
     def run_series(self, series, starts, win_start, win_end,
                    binsize=None, threshold=0.0, templates=None, raw=False):
@@ -319,12 +367,12 @@ class MatchedFilter:
         (nblocks, ntemplates, nbins) in caller order.  A later ``run()``
         requires ``set_data()`` again (python/matchedfilter/__init__.py:545-594).
         """
-        ser, layout, binsize, t0, nt = self._series_layout(series, starts, win_start, win_end,
-                                                           binsize, templates)
-        self._dataset = False
-        self._data_ready = set()
-        idx, val = self._dispatch_series(ser, layout, binsize, float(threshold), t0, nt)
-        return _format_result(idx, val, raw=raw)
+        ser, layout, width, t0, nt = self._series_layout(series, starts, win_start, win_end,
+                                                         binsize, templates)
+        self._has_data = False
+        self._books["data"].forget()
+        idx, val = self._dispatch_series(ser, layout, width, float(threshold), t0, nt)
+        return _result(idx, val, raw, None)
 
     def _dispatch_series(self, ser, layout, binsize, threshold, t0, nt):
         idx, val, _ = self._gpu.run_series(ser, layout.starts, layout.low, layout.high, t0, nt,
@@ -333,8 +381,19 @@ class MatchedFilter:
 
     # ---------------------------------------------------------- memory
     def empty_shared(self, shape, dtype=np.complex64):
-        """Host array suitable for the setters (ordinary NumPy storage)."""
-        return np.empty(shape, dtype=dtype)
+        """Host array for the setters, in page-locked memory the device reads
+        by DMA (the role of the reference's host-visible device storage,
+        python/matchedfilter/_shared.py:58-76).  It is an ordinary NumPy
+        array; PyTorch's pinned allocator backs it, and without PyTorch it is
+        plain NumPy storage."""
+        dt = np.dtype(dtype)
+        try:
+            import torch
+        except ImportError:
+            return np.empty(shape, dtype=dt)
+        nbytes = int(np.prod(shape, dtype=np.int64)) * dt.itemsize
+        buf = torch.empty(max(nbytes, 1), dtype=torch.uint8, pin_memory=True)
+        return buf.numpy()[:nbytes].view(dt).reshape(shape)
 
     def set_memory_limits(self, *, cache_bytes=None, series_bytes=None):
         """Set the dispatch-cache and series working-storage budgets in bytes."""

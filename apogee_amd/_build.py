@@ -29,7 +29,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmfhip.so")
 
-SOURCES = ["mfh_api.hip", "mfh_flat.hip", "mfh_hier.hip", "mfh_series.hip", "mfh_coarse.hip", "mfh_big.hip", "mfh_queue.hip", "mfh_grad.hip"]
+SOURCES = ["mfh_api.hip", "mfh_flat.hip", "mfh_hier.hip", "mfh_series.hip", "mfh_coarse.hip", "mfh_big.hip", "mfh_queue.hip", "mfh_grad.hip",
+           "mfh_probe.hip"]
 ARCH = os.environ.get("MFH_OFFLOAD_ARCH", "gfx950")
 
 
@@ -40,25 +41,46 @@ def hipcc():
     raise RuntimeError("hipcc not found; the HIP library cannot be built")
 
 
+FLAGS = ["--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=on",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form",
+         "-I", os.path.join(ROOT, "include"), "-I", CSRC]
+OBJDIR = os.path.join(ROOT, "_build")
+
+
 def command(out=LIB):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    return [hipcc(), "--offload-arch=%s" % ARCH, "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=on",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-mllvm", "-amdgpu-mfma-vgpr-form",
-            "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-            "-o", out] + srcs
+    """One hipcc line that compiles every source and links the library."""
+    return [hipcc()] + FLAGS + ["-shared", "-o", out] + [os.path.join(CSRC, s) for s in SOURCES]
 
 
-def build(verbose=False):
-    """Compile every source into libmfhip.so (always recompiles)."""
-    tmp = LIB + ".tmp%d" % os.getpid()
-    cmd = command(tmp)
+def _compile(src, obj):
+    res = subprocess.run([hipcc()] + FLAGS + ["-c", "-o", obj, os.path.join(CSRC, src)],
+                         capture_output=True, text=True)
+    return src, res
+
+
+def build(verbose=False, jobs=None):
+    """Compile every source into libmfhip.so (always recompiles): one hipcc
+    per translation unit in parallel, then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
     if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+        print(" ".join(command()), flush=True)
+    jobs = jobs or min(len(SOURCES), os.cpu_count() or 1, 16)
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        results = list(pool.map(_compile, SOURCES, objs))
+    failed = [(src, res) for src, res in results if res.returncode != 0]
+    for src, res in failed:
+        sys.stderr.write("%s:\n%s%s" % (src, res.stdout, res.stderr))
+    if failed:
+        raise RuntimeError("hipcc failed on %s" % ", ".join(src for src, _ in failed))
+    tmp = LIB + ".tmp%d" % os.getpid()
+    res = subprocess.run([hipcc(), "--offload-arch=%s" % ARCH, "-shared", "-fPIC", "-o", tmp] + objs,
+                         capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed (exit %d)" % res.returncode)
+        raise RuntimeError("hipcc link failed (exit %d)" % res.returncode)
     os.replace(tmp, LIB)
     return LIB
 

@@ -43,6 +43,8 @@ def _declare(lib):
     for name in ("mfh_set_data", "mfh_set_templates", "mfh_set_data_device_async",
                  "mfh_set_templates_device_async"):
         getattr(lib, name).argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp]
+    for name in ("mfh_bind_data", "mfh_bind_templates"):
+        getattr(lib, name).argtypes = [_vp, _vp]
     lib.mfh_nbins.restype = _c_size
     lib.mfh_nbins.argtypes = [_c_size, _c_size, _c_size, _c_size]
     lib.mfh_flat_peaks.restype = ctypes.c_long
@@ -70,6 +72,9 @@ def _declare(lib):
     lib.mfh_hier_coarse_max.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         _c_size, _c_size, ctypes.c_int, _vp]
     lib.mfh_selftest_mfma.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+    lib.mfh_probe_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    lib.mfh_probe_copy.argtypes = [ctypes.c_int, _c_size, ctypes.POINTER(ctypes.c_double)]
     lib.mfh_set_tuning.argtypes = [_vp, ctypes.c_char_p, ctypes.c_long]
     lib.mfh_prof_enable.argtypes = [_vp, ctypes.c_int]
     lib.mfh_prof_read.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
@@ -168,6 +173,23 @@ def selftest_mfma(device, variant, A, B):
     return C
 
 
+def probe_valu(device, op, chains, blocks=2048, iters=1024):
+    """Measured FP32 rate of independent scalar chains (op "fma" or "add"):
+    (flop per second, milliseconds)."""
+    ms = ctypes.c_double(0.0)
+    flop = ctypes.c_double(0.0)
+    check(lib().mfh_probe_valu(int(device), {"fma": 0, "add": 1}[op], int(chains), int(blocks), int(iters),
+                               ctypes.byref(ms), ctypes.byref(flop)), "mfh_probe_valu")
+    return flop.value / (ms.value * 1e-3), ms.value
+
+
+def probe_copy(device, nbytes=256 << 20):
+    """Measured device-to-device copy rate in bytes per second (read + write)."""
+    ms = ctypes.c_double(0.0)
+    check(lib().mfh_probe_copy(int(device), int(nbytes), ctypes.byref(ms)), "mfh_probe_copy")
+    return 2.0 * nbytes / (ms.value * 1e-3)
+
+
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
@@ -177,6 +199,82 @@ def _device_pointer(a):
     if hasattr(a, "data_ptr") and getattr(a, "is_cuda", False):
         return int(a.data_ptr())
     return None
+
+
+# ------------------------------------------------------------------ DLPack
+# The exchange structs of the DLPack standard (dlpack.h, v0.8 unversioned
+# capsule "dltensor"): what a producer hands over through __dlpack__().
+DLPACK_ROCM = 10
+_DL_COMPLEX = 5
+
+
+class _DLDevice(ctypes.Structure):
+    _fields_ = [("device_type", ctypes.c_int32), ("device_id", ctypes.c_int32)]
+
+
+class _DLDataType(ctypes.Structure):
+    _fields_ = [("code", ctypes.c_uint8), ("bits", ctypes.c_uint8), ("lanes", ctypes.c_uint16)]
+
+
+class _DLTensor(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("device", _DLDevice), ("ndim", ctypes.c_int32),
+                ("dtype", _DLDataType), ("shape", ctypes.POINTER(ctypes.c_int64)),
+                ("strides", ctypes.POINTER(ctypes.c_int64)), ("byte_offset", ctypes.c_uint64)]
+
+
+class _DLManagedTensor(ctypes.Structure):
+    _fields_ = [("dl_tensor", _DLTensor), ("manager_ctx", ctypes.c_void_p), ("deleter", ctypes.c_void_p)]
+
+
+_capsule_pointer = ctypes.pythonapi.PyCapsule_GetPointer
+_capsule_pointer.restype = ctypes.c_void_p
+_capsule_pointer.argtypes = [ctypes.py_object, ctypes.c_char_p]
+
+
+class DeviceView:
+    """A ROCm buffer borrowed from a DLPack producer.
+
+    ``capsule`` holds the producer's memory alive: while the view exists the
+    buffer stays valid (the producer's deleter runs when the capsule goes).
+    """
+    __slots__ = ("ptr", "shape", "complex64", "compact", "device", "capsule")
+
+    def __init__(self, ptr, shape, complex64, compact, device, capsule):
+        self.ptr, self.shape, self.complex64 = ptr, shape, complex64
+        self.compact, self.device, self.capsule = compact, device, capsule
+
+
+def device_view(producer, stream):
+    """Borrow ``producer``'s ROCm buffer for work queued on ``stream``.
+
+    ``producer.__dlpack__(stream=stream)`` is the DLPack hand-over: the
+    producer orders ``stream`` after its own pending writes (PyTorch records
+    an event on its current stream and makes ``stream`` wait on it), so no
+    host synchronisation is needed.  A PyTorch tensor is detached and its
+    lazy conjugation resolved first (DLPack carries neither).
+    """
+    try:
+        import torch
+        if isinstance(producer, torch.Tensor):
+            producer = producer.detach().resolve_conj()
+    except ImportError:
+        pass
+    capsule = producer.__dlpack__(stream=int(stream))
+    addr = _capsule_pointer(capsule, b"dltensor")
+    t = _DLManagedTensor.from_address(addr).dl_tensor
+    if t.device.device_type != DLPACK_ROCM:
+        raise TypeError("DLPack producer handed over device type %d, not ROCm" % t.device.device_type)
+    shape = tuple(int(t.shape[i]) for i in range(t.ndim))
+    compact = True
+    if t.strides:
+        expect = 1
+        for axis in range(t.ndim - 1, -1, -1):
+            if shape[axis] != 1 and int(t.strides[axis]) != expect:
+                compact = False
+            expect *= shape[axis]
+    c64 = (t.dtype.code, t.dtype.bits, t.dtype.lanes) == (_DL_COMPLEX, 64, 1)
+    return DeviceView(int(t.data or 0) + int(t.byte_offset), shape, c64, compact,
+                      int(t.device.device_id), capsule)
 
 
 class Context:
@@ -197,6 +295,7 @@ class Context:
             raise RuntimeError("mfh_create failed: %s" % last_error())
         self._h = ctypes.c_void_p(h)
         self.last_refinements = 0
+        self._bound = {}
 
     # ------------------------------------------------------------ lifetime
     def destroy(self):
@@ -224,24 +323,38 @@ class Context:
 
     # -------------------------------------------------------------- ingest
     def set_rows(self, kind, row0, spectra):
-        """Copy spectra into data (kind="data") or template rows from row0.
-
-        ``spectra`` is a C-contiguous complex64 host array (rows, n), or a
-        complex64 ROCm tensor of the same shape (copied device to device).
-        """
-        dev = _device_pointer(spectra)
-        rows = int(spectra.shape[0])
+        """Copy host spectra, a C-contiguous complex64 array (rows, n), into
+        data (kind="data") or template rows from row0; returns when the copy
+        is done."""
         L = lib()
-        if dev is not None:
-            # the producer's work runs on PyTorch's stream: finish it first
-            import torch
-            torch.cuda.current_stream(spectra.device).synchronize()
-            fn = L.mfh_set_data_device_async if kind == "data" else L.mfh_set_templates_device_async
-            check(fn(self.handle, int(row0), rows, ctypes.c_void_p(dev)), "set %s" % kind)
-            self.sync()
-            return
         fn = L.mfh_set_data if kind == "data" else L.mfh_set_templates
-        check(fn(self.handle, int(row0), rows, _ptr(spectra)), "set %s" % kind)
+        check(fn(self.handle, int(row0), int(spectra.shape[0]), _ptr(spectra)), "set %s" % kind)
+        self._bound.pop(kind, None)
+
+    def copy_rows(self, kind, row0, view):
+        """Copy a borrowed ROCm buffer (:class:`DeviceView`, rows x n
+        complex64) into rows from row0, device to device; returns when the
+        copy is done, so the producer may reuse its buffer."""
+        L = lib()
+        fn = L.mfh_set_data_device_async if kind == "data" else L.mfh_set_templates_device_async
+        check(fn(self.handle, int(row0), int(view.shape[0]), ctypes.c_void_p(view.ptr)), "set %s" % kind)
+        self.sync()
+        self._bound.pop(kind, None)
+
+    def bind_rows(self, kind, view):
+        """Read the whole data or template bank in place from a borrowed ROCm
+        buffer (:class:`DeviceView` of shape (rows, n), complex64, compact).
+        Nothing is copied; the view is kept until the bank is replaced, and
+        the library never writes the buffer (a later single-row update first
+        copies the bank into the context's own storage)."""
+        L = lib()
+        fn = L.mfh_bind_data if kind == "data" else L.mfh_bind_templates
+        check(fn(self.handle, ctypes.c_void_p(view.ptr)), "bind %s" % kind)
+        self._bound[kind] = view
+
+    def borrow(self, producer):
+        """:func:`device_view` of ``producer`` ordered on this context's stream."""
+        return device_view(producer, self.stream())
 
     # ---------------------------------------------------------------- runs
     def peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, counts=False):
@@ -310,11 +423,9 @@ class Context:
                    coarse_threshold=None):
         """Block spectra of a series on the device, then the flat (or, with a
         coarse threshold, hierarchical) filter; returns (index, value, refined)."""
-        dev = _device_pointer(series)
-        if dev is not None:
-            import torch
-            torch.cuda.current_stream(series.device).synchronize()
-            sp, ns, on_dev = ctypes.c_void_p(dev), int(series.numel()), 1
+        view = self.borrow(series) if _device_pointer(series) is not None else None
+        if view is not None:
+            sp, ns, on_dev = ctypes.c_void_p(view.ptr), int(view.shape[0]), 1
         else:
             sp, ns, on_dev = _ptr(series), int(series.size), 0
         st = np.ascontiguousarray(starts, dtype=np.uintp)
@@ -354,16 +465,15 @@ class Context:
 
     def series_forward(self, series, starts, row0=0):
         """Block spectra of a series into data rows [row0, row0 + len(starts))."""
-        dev = _device_pointer(series)
-        if dev is not None:
-            import torch
-            torch.cuda.current_stream(series.device).synchronize()
-            sp, ns, on_dev = ctypes.c_void_p(dev), int(series.numel()), 1
+        view = self.borrow(series) if _device_pointer(series) is not None else None
+        if view is not None:
+            sp, ns, on_dev = ctypes.c_void_p(view.ptr), int(view.shape[0]), 1
         else:
             sp, ns, on_dev = _ptr(series), int(series.size), 0
         st = np.ascontiguousarray(starts, dtype=np.uintp)
         check(lib().mfh_series_forward(self.handle, sp, ns, on_dev, _ptr(st), int(st.size), int(row0)),
               "mfh_series_forward")
+        self._bound.pop("data", None)      # the library wrote data rows of its own copy
 
     # ------------------------------------------------------------- tuning
     def set_tuning(self, key, value):
@@ -412,13 +522,11 @@ class Queue:
 
     def set_pool(self, pool):
         """pool: 1-D complex64 host array (copied) or ROCm tensor (referenced)."""
-        dev = _device_pointer(pool)
-        if dev is not None:
-            import torch
-            torch.cuda.current_stream(pool.device).synchronize()
-            check(lib().mfh_queue_set_pool(self.handle, ctypes.c_void_p(dev), int(pool.numel()), 1),
+        if _device_pointer(pool) is not None:
+            view = device_view(pool, self.stream())
+            check(lib().mfh_queue_set_pool(self.handle, ctypes.c_void_p(view.ptr), int(view.shape[0]), 1),
                   "mfh_queue_set_pool")
-            self._pool_ref = pool     # the queue reads it in place
+            self._pool_ref = view     # the queue reads it in place
         else:
             check(lib().mfh_queue_set_pool(self.handle, _ptr(pool), int(pool.size), 0),
                   "mfh_queue_set_pool")

@@ -93,10 +93,10 @@ class PeakFilter:
 
     # ------------------------------------------------------------ kernels
     def _forward(self, data, templates, binsize, threshold, start, end):
-        d = data.detach().contiguous()
-        t = templates.detach().contiguous()
-        self._ctx.set_rows("data", 0, d)            # waits for torch's stream first
-        self._ctx.set_rows("template", 0, t)
+        # read in place: the banks are bound to the tensors' memory, and the
+        # context's stream is ordered after PyTorch's by the DLPack hand-over
+        self._ctx.bind_rows("data", self._ctx.borrow(data.detach().contiguous()))
+        self._ctx.bind_rows("template", self._ctx.borrow(templates.detach().contiguous()))
         nb = -(-(end - start) // binsize)
         shape = (self.ndata, self.ntemplates, nb)
         index = torch.empty(shape, dtype=torch.int64, device=self._torch_device)
@@ -112,12 +112,14 @@ class PeakFilter:
         g = gvalue.to(torch.complex64).contiguous()
         gd = torch.empty_like(d) if need_d else None
         gt = torch.empty_like(t) if need_t else None
-        torch.cuda.current_stream(self._torch_device).synchronize()
+        # order the context's stream after PyTorch's pending work on the inputs
+        keep = [self._ctx.borrow(x) for x in (d, t, g, index)]
         self._ctx.peak_grad_async(d.data_ptr(), self.ndata, t.data_ptr(), self.ntemplates, index.shape[2],
                                   index.data_ptr(), g.data_ptr(),
                                   gd.data_ptr() if gd is not None else 0,
                                   gt.data_ptr() if gt is not None else 0)
         self._ctx.sync()
+        del keep
         return gd, gt
 
     def recompute(self, data, templates, index):
@@ -128,10 +130,11 @@ class PeakFilter:
         t = templates.detach().contiguous()
         idx = index.contiguous()
         out = torch.empty(idx.shape, dtype=torch.complex64, device=self._torch_device)
-        torch.cuda.current_stream(self._torch_device).synchronize()
+        keep = [self._ctx.borrow(x) for x in (d, t, idx)]
         self._ctx.peak_values_async(d.data_ptr(), self.ndata, t.data_ptr(), self.ntemplates, idx.shape[2],
                                     idx.data_ptr(), out.data_ptr())
         self._ctx.sync()
+        del keep
         return out
 
 

@@ -83,6 +83,18 @@ void *Ctx::scratch(const char *name, size_t bytes) {
   return b.p;
 }
 
+int Ctx::own_bank(bool templates) {
+  cf *&cur = templates ? tmpl : data;
+  cf *own = templates ? tmpl_own : data_own;
+  if (cur == own) return 0;
+  const size_t bytes = (size_t)(templates ? nt : nd) * n * sizeof(cf);
+  hipSetDevice(dev);
+  if (!hip_ok(hipMemcpyAsync(own, cur, bytes, hipMemcpyDeviceToDevice, stream), "copy bound bank")) return -1;
+  cur = own;
+  if (templates) coarse_dirty = true;
+  return 0;
+}
+
 hipEvent_t Ctx::take_event() {
   if (!event_pool.empty()) {
     hipEvent_t e = event_pool.back();
@@ -176,8 +188,8 @@ extern "C" mfh_ctx *mfh_create(int device, size_t n, int nd, int nt) {
     delete c;
     return nullptr;
   }
-  c->data = (cf *)c->scratch("data", (size_t)nd * n * sizeof(cf));
-  c->tmpl = (cf *)c->scratch("tmpl", (size_t)nt * n * sizeof(cf));
+  c->data = c->data_own = (cf *)c->scratch("data", (size_t)nd * n * sizeof(cf));
+  c->tmpl = c->tmpl_own = (cf *)c->scratch("tmpl", (size_t)nt * n * sizeof(cf));
   if (!c->data || !c->tmpl || (n <= 16384 && !c->twiddles(n))) {
     delete c;
     return nullptr;
@@ -205,11 +217,26 @@ extern "C" int mfh_sync(mfh_ctx *ctx) {
   return hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize") ? 0 : -1;
 }
 
-static int set_rows(Ctx *c, cf *base, int total, int row0, int rows, const void *src, hipMemcpyKind kind,
-                    bool sync) {
+// Rows [row0, row0 + rows) of the data (templates = false) or template bank
+// from `src`.  A bank bound to a caller's buffer is first copied into the
+// context's own allocation, so the library never writes the caller's memory.
+static int set_rows(Ctx *c, bool templates, int row0, int rows, const void *src, hipMemcpyKind kind, bool sync) {
+  const int total = templates ? c->nt : c->nd;
   if (row0 < 0 || rows < 1 || row0 + rows > total) return fail("rows [%d, %d) out of range [0, %d)", row0, row0 + rows, total);
   if (!src) return fail("null spectrum pointer");
   hipSetDevice(c->dev);
+  const bool whole = row0 == 0 && rows == total;
+  if (whole) {
+    // every row is replaced: nothing of a bound bank needs to survive
+    if (templates)
+      c->tmpl = c->tmpl_own;
+    else
+      c->data = c->data_own;
+  } else if (c->own_bank(templates) < 0) {
+    return -1;
+  }
+  if (templates) c->coarse_dirty = true;
+  cf *base = templates ? c->tmpl : c->data;
   if (!hip_ok(hipMemcpyAsync(base + (size_t)row0 * c->n, src, (size_t)rows * c->n * sizeof(cf), kind, c->stream),
               "hipMemcpyAsync"))
     return -1;
@@ -219,25 +246,45 @@ static int set_rows(Ctx *c, cf *base, int total, int row0, int rows, const void 
 
 extern "C" int mfh_set_data(mfh_ctx *ctx, int row0, int rows, const void *host_c64) {
   if (!ctx) return fail("null context");
-  Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  return set_rows(c, c->data, c->nd, row0, rows, host_c64, hipMemcpyHostToDevice, true);
+  return set_rows(reinterpret_cast<Ctx *>(ctx), false, row0, rows, host_c64, hipMemcpyHostToDevice, true);
 }
 extern "C" int mfh_set_templates(mfh_ctx *ctx, int row0, int rows, const void *host_c64) {
   if (!ctx) return fail("null context");
-  Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  c->coarse_dirty = true;
-  return set_rows(c, c->tmpl, c->nt, row0, rows, host_c64, hipMemcpyHostToDevice, true);
+  return set_rows(reinterpret_cast<Ctx *>(ctx), true, row0, rows, host_c64, hipMemcpyHostToDevice, true);
 }
 extern "C" int mfh_set_data_device_async(mfh_ctx *ctx, int row0, int rows, const void *dev_c64) {
   if (!ctx) return fail("null context");
-  Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  return set_rows(c, c->data, c->nd, row0, rows, dev_c64, hipMemcpyDeviceToDevice, false);
+  return set_rows(reinterpret_cast<Ctx *>(ctx), false, row0, rows, dev_c64, hipMemcpyDeviceToDevice, false);
 }
 extern "C" int mfh_set_templates_device_async(mfh_ctx *ctx, int row0, int rows, const void *dev_c64) {
   if (!ctx) return fail("null context");
-  Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  c->coarse_dirty = true;
-  return set_rows(c, c->tmpl, c->nt, row0, rows, dev_c64, hipMemcpyDeviceToDevice, false);
+  return set_rows(reinterpret_cast<Ctx *>(ctx), true, row0, rows, dev_c64, hipMemcpyDeviceToDevice, false);
+}
+
+// Point a bank at the caller's device buffer (read in place, never written).
+static int bind_bank(Ctx *c, bool templates, const void *dev_c64) {
+  if (!dev_c64) return fail("null spectrum pointer");
+  if ((uintptr_t)dev_c64 % alignof(cf)) return fail("bound spectra must be 8-byte aligned");
+  hipPointerAttribute_t attr;
+  if (!hip_ok(hipPointerGetAttributes(&attr, dev_c64), "hipPointerGetAttributes")) return -1;
+  if (attr.type != hipMemoryTypeDevice || attr.device != c->dev)
+    return fail("bound spectra must be device memory of device %d", c->dev);
+  if (templates) {
+    c->tmpl = (cf *)dev_c64;
+    c->coarse_dirty = true;
+  } else {
+    c->data = (cf *)dev_c64;
+  }
+  return 0;
+}
+
+extern "C" int mfh_bind_data(mfh_ctx *ctx, const void *dev_c64) {
+  if (!ctx) return fail("null context");
+  return bind_bank(reinterpret_cast<Ctx *>(ctx), false, dev_c64);
+}
+extern "C" int mfh_bind_templates(mfh_ctx *ctx, const void *dev_c64) {
+  if (!ctx) return fail("null context");
+  return bind_bank(reinterpret_cast<Ctx *>(ctx), true, dev_c64);
 }
 
 extern "C" size_t mfh_nbins(size_t n, size_t binsize, size_t ws, size_t we) {

@@ -19,8 +19,20 @@ struct Ctx {
   size_t n = 0;
   int nd = 0, nt = 0;
   hipStream_t stream = nullptr;
-  cf *data = nullptr;  // [nd][n]
-  cf *tmpl = nullptr;  // [nt][n]
+  // The banks every launch reads: [nd][n] and [nt][n].  Each points either at
+  // the context's own allocation (data_own / tmpl_own) or, after
+  // mfh_bind_data / mfh_bind_templates, at a caller's device buffer that is
+  // read in place and never written.
+  cf *data = nullptr;
+  cf *tmpl = nullptr;
+  cf *data_own = nullptr;
+  cf *tmpl_own = nullptr;
+  bool data_bound() const { return data != data_own; }
+  bool tmpl_bound() const { return tmpl != tmpl_own; }
+  // Before the library writes rows of a bound bank: copy the caller's bank
+  // into the context's own allocation (queued on the stream) and read that
+  // from then on, so the caller's buffer is never modified.
+  int own_bank(bool templates);
   int tune_dpw = 0;          // 0: automatic
   // hierarchical state: coarse band, bank-wide reference fraction (< 0: per
   // template), scaled coarse templates [nt][band], rebuilt when dirty

@@ -337,6 +337,7 @@ extern "C" int mfh_series_forward(mfh_ctx *ctx, const void *series_c64, size_t n
   if (!hip_ok(hipMemcpyAsync(dstart, st.data(), nblocks * sizeof(long long), hipMemcpyHostToDevice, c->stream),
               "copy starts"))
     return -1;
+  if (c->own_bank(false) < 0) return -1;
   if (mfh::launch_series_forward(c, dser, nseries, dstart, nblocks, c->data + (size_t)row0 * c->n) < 0) return -1;
   return hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize") ? 0 : -1;
 }

@@ -19,7 +19,7 @@ table and the coarse gate from the reference profile through
 """
 import numpy as np
 
-from . import MatchedFilter, _from_any
+from . import MatchedFilter, _host_view
 
 #: Coarse-screen precisions: name -> (code of mfh_hier_set_coarse_precision,
 #: default gate widening).  The widening bounds the relative error of the
@@ -80,7 +80,7 @@ class HierarchicalFilter(MatchedFilter):
     def set_reference(self, power):
         """Reference output-power profile (length n, any scale), or None."""
         if power is not None:
-            p = np.ascontiguousarray(_from_any(power), dtype=np.float32)
+            p = np.ascontiguousarray(_host_view(power), dtype=np.float32)
             if p.shape != (self.n,):
                 raise ValueError("reference must be a one-dimensional array of length %d" % self.n)
             if not np.isfinite(p).all() or np.any(p < 0) or not np.any(p > 0):

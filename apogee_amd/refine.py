@@ -92,16 +92,16 @@ class RefineQueue:
         ``template_offset[r] + j*n[r]``.  ``binsize`` defaults to n[r];
         ``window`` is None, one (start, end) pair, or an (nreq, 2) array.
         """
-        from . import _from_any, _is_device_tensor
-        p = _from_any(pool)
-        if _is_device_tensor(p):
+        from . import _host_view, _on_rocm
+        if _on_rocm(pool):
             import torch
+            p = torch.from_dlpack(pool) if not isinstance(pool, torch.Tensor) else pool
             if p.dtype != torch.complex64 or p.dim() != 1:
                 raise ValueError("pool must be a 1-D complex64 tensor")
             p = p.contiguous()
             size = int(p.numel())
         else:
-            p = np.ascontiguousarray(p, dtype=np.complex64)
+            p = np.ascontiguousarray(_host_view(pool), dtype=np.complex64)
             if p.ndim != 1:
                 raise ValueError("pool must be 1-D, got shape %s" % (p.shape,))
             size = int(p.size)

@@ -77,11 +77,31 @@ int mfh_set_templates(mfh_ctx *ctx, int row0, int rows, const void *host_c64);
 
 /* Device-resident variants: `dev_c64` is a device pointer on the context's
  * device (e.g. a torch tensor); copied device to device, queued on the stream.
+ * Used for single rows; a whole bank is bound instead (below).
  * Replaces: nothing in the reference (it refuses device arrays,
- * python/matchedfilter/__init__.py:103-137); listed as the zero-copy ingest
- * follow-up of the hot path. */
+ * python/matchedfilter/__init__.py:103-137); the device-resident half of the
+ * ingest row of the hot path. */
 int mfh_set_data_device_async(mfh_ctx *ctx, int row0, int rows, const void *dev_c64);
 int mfh_set_templates_device_async(mfh_ctx *ctx, int row0, int rows, const void *dev_c64);
+
+/* Zero-copy ingest: every later run reads the whole data (or template) bank
+ * in place from `dev_c64`, nd x n (nt x n) complex64, 8-byte aligned device
+ * memory of the context's device (checked with hipPointerGetAttributes).
+ * Nothing is copied and nothing is queued: the caller orders the context's
+ * stream (mfh_stream) after its own writes to the buffer (a DLPack
+ * hand-over does exactly that) and keeps the buffer alive and unchanged
+ * until the bank is replaced or the context destroyed.  The library never
+ * writes it: a later single-row set, or a series forward into the data rows,
+ * first copies the bank into the context's own storage.  Replacing the
+ * whole bank (mfh_set_* with row0 0 and every row, or another bind) ends the
+ * binding.
+ * Replaces: the reference's retained caller spectrum of the hierarchical CPU
+ * plan (src/hmf.c:139-143, held by python/matchedfilter/__init__.py:338-353)
+ * and, for device memory, its refusal of accelerator arrays
+ * (python/matchedfilter/__init__.py:103-131); host-visible shared storage is
+ * its GPU alternative (python/matchedfilter/_shared.py:58-76). */
+int mfh_bind_data(mfh_ctx *ctx, const void *dev_c64);
+int mfh_bind_templates(mfh_ctx *ctx, const void *dev_c64);
 
 /* Number of bins of window [ws, we) at `binsize` (0 if empty).
  * Replaces: ap_mf_nbins (matchedfilter.h:56; src/matchfilt.c:151-156). */
@@ -155,6 +175,21 @@ int mfh_hier_coarse_max(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t ws,
  * C is 16x16.  Pins the fragment layouts the coarse screen relies on.
  * Replaces: nothing in the reference (a self-test of this build). */
 int mfh_selftest_mfma(int device, int variant, const float *A, const float *B, float *C);
+
+/* Measured VALU ceiling: `blocks` workgroups of 256 lanes each run `chains`
+ * (4, 8, 16 or 32) independent chains of iters x 16 dependent scalar FP32
+ * operations, op 0 = FMA (2 flop) or 1 = add (1 flop); no packed FP32.
+ * Writes the second of two identical launches' duration (ms, HIP events) and
+ * its flop count.  Synchronous, on a private stream.
+ * Replaces: the FMA / add chain probe of tools/gpu_roofline.py:186-233 (the
+ * add rate is "the one that actually bounds an FFT", :225-233). */
+int mfh_probe_valu(int device, int op, int chains, int blocks, int iters, double *ms_out,
+                   double *flop_out);
+
+/* Measured copy bandwidth: one grid-stride float4 copy of `bytes` (a
+ * multiple of 16), device to device; writes the second launch's duration.
+ * Replaces: the COPY probe of tools/gpu_roofline.py:236-243. */
+int mfh_probe_copy(int device, size_t bytes, double *ms_out);
 
 /* Same with device outputs, queued on the stream; if host_refined is not
  * NULL it receives the survivor count once the stream reaches that point.

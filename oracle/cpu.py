@@ -16,26 +16,41 @@ def build():
     return LIB
 
 
-def load():
+def build_native(outdir):
+    """Build the same source with -march=native for the host it runs on (the
+    timed CPU baseline) into `outdir`; returns the library path."""
+    out = os.path.join(outdir, "libmfcpu_native.so")
+    subprocess.run(["g++", "-O3", "-march=native", "-ffp-contract=off", "-fPIC", "-std=c++17", "-shared",
+                    "-o", out, os.path.join(HERE, "mfcpu.cpp")], check=True)
+    return out
+
+
+def _declare(L):
+    L.mfcpu_flat.restype = ctypes.c_long
+    L.mfcpu_flat.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                             ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_size_t,
+                             ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.mfcpu_block_spectrum.restype = ctypes.c_int
+    L.mfcpu_block_spectrum.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_size_t, ctypes.c_void_p]
+    return L
+
+
+def load(path=None):
+    """The checker library (default), or the library at `path`."""
     global _lib
+    if path is not None:
+        return _declare(ctypes.CDLL(path))
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = ctypes.CDLL(LIB)
-        L.mfcpu_flat.restype = ctypes.c_long
-        L.mfcpu_flat.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
-                                 ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_size_t,
-                                 ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
-        L.mfcpu_block_spectrum.restype = ctypes.c_int
-        L.mfcpu_block_spectrum.argtypes = [ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
-                                           ctypes.c_size_t, ctypes.c_void_p]
-        _lib = L
+        _lib = _declare(ctypes.CDLL(LIB))
     return _lib
 
 
-def flat(D, T, binsize=None, threshold=0.0, window=None):
+def flat(D, T, binsize=None, threshold=0.0, window=None, lib=None):
     """FP32 CPU flat filter: (idx int64, val complex64, counts int32)."""
-    L = load()
+    L = load() if lib is None else lib
     D = np.ascontiguousarray(np.atleast_2d(D), dtype=np.complex64)
     T = np.ascontiguousarray(np.atleast_2d(T), dtype=np.complex64)
     nd, n = D.shape

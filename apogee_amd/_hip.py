@@ -45,6 +45,8 @@ def _declare(lib):
         getattr(lib, name).argtypes = [_vp, ctypes.c_int, ctypes.c_int, _vp]
     for name in ("mfh_bind_data", "mfh_bind_templates"):
         getattr(lib, name).argtypes = [_vp, _vp]
+    lib.mfh_get_rows.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
+    lib.mfh_hier_set_coarse_templates.argtypes = [_vp, _c_size, _vp, ctypes.c_int]
     lib.mfh_nbins.restype = _c_size
     lib.mfh_nbins.argtypes = [_c_size, _c_size, _c_size, _c_size]
     lib.mfh_flat_peaks.restype = ctypes.c_long
@@ -337,7 +339,8 @@ class Context:
         copy is done, so the producer may reuse its buffer."""
         L = lib()
         fn = L.mfh_set_data_device_async if kind == "data" else L.mfh_set_templates_device_async
-        check(fn(self.handle, int(row0), int(view.shape[0]), ctypes.c_void_p(view.ptr)), "set %s" % kind)
+        rows = int(np.prod(view.shape, dtype=np.int64)) // self.n
+        check(fn(self.handle, int(row0), rows, ctypes.c_void_p(view.ptr)), "set %s" % kind)
         self.sync()
         self._bound.pop(kind, None)
 

@@ -261,6 +261,20 @@ extern "C" int mfh_set_templates_device_async(mfh_ctx *ctx, int row0, int rows, 
   return set_rows(reinterpret_cast<Ctx *>(ctx), true, row0, rows, dev_c64, hipMemcpyDeviceToDevice, false);
 }
 
+extern "C" int mfh_get_rows(mfh_ctx *ctx, int templates, int row0, int rows, void *host_c64) {
+  if (!ctx || !host_c64) return fail("null argument");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  const int total = templates ? c->nt : c->nd;
+  if (row0 < 0 || rows < 1 || row0 + rows > total) return fail("rows [%d, %d) out of range [0, %d)", row0, row0 + rows, total);
+  hipSetDevice(c->dev);
+  const cf *base = templates ? c->tmpl : c->data;
+  if (!hip_ok(hipMemcpyAsync(host_c64, base + (size_t)row0 * c->n, (size_t)rows * c->n * sizeof(cf),
+                             hipMemcpyDeviceToHost, c->stream),
+              "hipMemcpyAsync"))
+    return -1;
+  return hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize") ? 0 : -1;
+}
+
 // Point a bank at the caller's device buffer (read in place, never written).
 static int bind_bank(Ctx *c, bool templates, const void *dev_c64) {
   if (!dev_c64) return fail("null spectrum pointer");

@@ -246,6 +246,29 @@ extern "C" int mfh_hier_set_band(mfh_ctx *ctx, size_t band, double ref_fraction)
   return 0;
 }
 
+extern "C" int mfh_hier_set_coarse_templates(mfh_ctx *ctx, size_t band, const void *ct0_c64, int on_device) {
+  if (!ctx || !ct0_c64) return fail("null argument");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (band < 64 || band >= c->n || (band & (band - 1)) || !mfh::flat_supported(band))
+    return fail("band must be a power of two, >= 64 and < n (got %zu)", band);
+  hipSetDevice(c->dev);
+  const size_t bytes = (size_t)c->nt * band * sizeof(mfh::cf);
+  mfh::cf *ct = (mfh::cf *)c->scratch("coarse_tmpl", bytes);
+  if (!ct) return -1;
+  if (!hip_ok(hipMemcpyAsync(ct, ct0_c64, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                             c->stream),
+              "copy coarse templates"))
+    return -1;
+  c->coarse_band = (int)band;
+  // a later template change rebuilds from the spectra, each template by its
+  // own band fraction (the reference's rule when no profile is set)
+  c->coarse_ref_f = -1.0;
+  c->coarse_tmpl = ct;
+  c->coarse_dirty = false;
+  if (mfh::coarse_mfma_prepare(c) < 0) return -1;
+  return hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize") ? 0 : -1;
+}
+
 extern "C" int mfh_hier_set_coarse_precision(mfh_ctx *ctx, int precision, float delta) {
   if (!ctx) return fail("null context");
   Ctx *c = reinterpret_cast<Ctx *>(ctx);

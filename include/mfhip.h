@@ -103,6 +103,14 @@ int mfh_set_templates_device_async(mfh_ctx *ctx, int row0, int rows, const void 
 int mfh_bind_data(mfh_ctx *ctx, const void *dev_c64);
 int mfh_bind_templates(mfh_ctx *ctx, const void *dev_c64);
 
+/* Copy resident rows back to host memory: data rows (templates = 0) or
+ * template rows (1), [row0, row0 + rows), as the kernels read them (a bound
+ * bank reads from the caller's buffer).  Synchronous.
+ * Replaces: reading the host-visible spectra a forward pass filled
+ * (Context.forward into empty_shared storage,
+ * python/matchedfilter/_vkcompute.py:855-903). */
+int mfh_get_rows(mfh_ctx *ctx, int templates, int row0, int rows, void *host_c64);
+
 /* Number of bins of window [ws, we) at `binsize` (0 if empty).
  * Replaces: ap_mf_nbins (matchedfilter.h:56; src/matchfilt.c:151-156). */
 size_t mfh_nbins(size_t n, size_t binsize, size_t ws, size_t we);
@@ -138,6 +146,18 @@ int mfh_flat_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
  * host-side coarse-template scaling of HierarchicalFilter._gpu_dispatch
  * (python/matchedfilter/__init__.py:1284-1303). */
 int mfh_hier_set_band(mfh_ctx *ctx, size_t band, double ref_fraction);
+
+/* Caller-scaled coarse templates: ct0 is nt x band complex64 (host memory,
+ * or device memory of the context's device when on_device is 1), already
+ * T[:band] / sqrt(f) by the caller's normalisation.  Sets the band and uses
+ * ct0 as the coarse bank as given (no device scaling) until the band or the
+ * templates change; a later template change rebuilds the coarse bank from
+ * the spectra with per-template band fractions.  Synchronous.
+ * Replaces: the ct0 argument of Context.hier_peaks
+ * (python/matchedfilter/_vkcompute.py:534-535, uploaded at :608-613), which
+ * HierarchicalFilter._gpu_dispatch scales on the host
+ * (python/matchedfilter/__init__.py:1284-1303). */
+int mfh_hier_set_coarse_templates(mfh_ctx *ctx, size_t band, const void *ct0_c64, int on_device);
 
 /* Hierarchical run: coarse band screen (the band-point flat filter on
  * D[:band] and the coarse templates, one bin over the coarse window), gate

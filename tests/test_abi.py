@@ -52,3 +52,18 @@ def test_supported_lengths_query():
         assert _hip.supported(n)
     for n in (0, 32, 1000, 3 << 16, 1 << 21):
         assert not _hip.supported(n)
+
+
+def test_integration_binding_is_the_tested_shim():
+    """INTEGRATION.md shows tests/hip_context.py verbatim, the binding that
+    tests/test_gpu_context_shim.py runs on the GPU."""
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    blocks = re.findall(r"```python\n(.*?)```", doc, flags=re.S)
+    shim = open(os.path.join(ROOT, "tests", "hip_context.py")).read()
+    assert shim in blocks
+
+
+def test_shim_binds_only_declared_entry_points():
+    text = open(os.path.join(ROOT, "tests", "hip_context.py")).read()
+    used = set(re.findall(r"\b(mfh_\w+)\b", text))
+    assert used and used <= set(declared()), used - set(declared())

@@ -145,7 +145,7 @@ class _RowBook:
     def record(self, row):
         if row is None:
             self._full = True
-        else:
+        elif not self._full:
             self._have[row] = True
             self._full = bool(self._have.all())
 

@@ -277,7 +277,7 @@ def hier_leg(args):
     out = {"workload": "chirp bank f^-7/6 on bins 20..1500 (unit norm), complex N(0,1) data "
                        "spectra (|z|^2 of mean 2, the gate model's units), n=4096, 16x1024, "
                        "snr 5.5, fd 1e-2, band 512; timed on noise-only data (as the reference's "
-                       "published hierarchical figures are) and on data with 1% of pairs injected "
+                       "published hierarchical figures are) and on data with 1%% of pairs injected "
                        "at SNR 8; false dismissal over %d single SNR-8 injections per screen, and "
                        "over as many at the calibration point SNR 5.5" % args.fd_injections,
            "flat_pairs_per_s": pairs / t_flat, "flat_kernel_us": st_f["flat"], "screens": {}}

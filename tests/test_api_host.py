@@ -113,6 +113,13 @@ def test_single_rows_track_readiness(fake):
     f.set_data(np.ones(n, np.complex64), index=1)
     f.set_data(np.ones(n, np.complex64), index=2)
     f.run()
+    # a whole bank stays ready when single rows are replaced afterwards
+    g = mf.MatchedFilter(n, 3, 2)
+    g.set_templates(np.ones((2, n), np.complex64))
+    g.set_data(np.ones((3, n), np.complex64))
+    g.set_data(np.zeros(n, np.complex64), index=1)
+    g.set_templates(np.zeros(n, np.complex64), index=0)
+    g.run()
 
 
 def test_device_arrays_other_than_rocm_are_refused(fake):

@@ -63,8 +63,15 @@ const cf *Ctx::twiddles(size_t len) {
 }
 
 void *Ctx::scratch(const char *name, size_t bytes) {
+  auto found = bufs.find(name);
+  if (found != bufs.end() && found->second.bytes >= bytes && found->second.p) return found->second.p;
+  if (no_alloc) {
+    fail("an allocation-free call needs the context's '%s' buffer (%zu bytes), which is not built yet: "
+         "run mfh_hier_peaks or mfh_hier_peaks_async with this band and precision once first",
+         name, bytes);
+    return nullptr;
+  }
   Buf &b = bufs[name];
-  if (b.bytes >= bytes && b.p) return b.p;
   if (b.p) {
     hipStreamSynchronize(stream);
     hipFree(b.p);
@@ -108,13 +115,13 @@ hipEvent_t Ctx::take_event() {
 
 // Event pair around one launch, when profiling is on (events are pooled).
 Ctx::ProfScope::ProfScope(Ctx *c, const char *name) : c_(c), name_(name) {
-  if (!c_->prof_on || c_->capturing) return;
+  if (!c_->prof_on || c_->capturing || c_->no_alloc) return;
   ev_.a = c_->take_event();
   ev_.b = c_->take_event();
   hipEventRecord(ev_.a, c_->stream);
 }
 Ctx::ProfScope::~ProfScope() {
-  if (!c_->prof_on || c_->capturing) return;
+  if (!c_->prof_on || c_->capturing || c_->no_alloc) return;
   hipEventRecord(ev_.b, c_->stream);
   ev_.name = name_;
   c_->prof_events.push_back(ev_);

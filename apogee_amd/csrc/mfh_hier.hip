@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "mfh_dft.h"
@@ -95,6 +97,12 @@ __global__ void gate_kernel(const int64_t *cidx, const cf *cval, int nd, int nt,
   }
 }
 
+// Clears `n` ints (n <= 64), one workgroup: a kernel rather than a memset so
+// that a captured sequence holds no memset node.
+__global__ void zero_ints_kernel(int *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 }  // namespace
 
 int hier_prepare(Ctx *c) {
@@ -133,7 +141,7 @@ int *hier_counter(Ctx *c, int parity) {
 }
 
 int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,
-             float coarse_thr, int64_t *idx, cf *val, int *count_out, const cf *data_rows) {
+             float coarse_thr, int64_t *idx, cf *val, int *count_out, const cf *data_rows, const HierScratch *hs) {
   if (c->coarse_band <= 0) return fail("no coarse band: call mfh_hier_set_band first");
   if (!(coarse_thr >= 0.f) || !std::isfinite(coarse_thr))
     return fail("coarse threshold must be finite and nonnegative");
@@ -151,11 +159,27 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   const size_t R = n / m;
   size_t cs = ws / R, ce = std::min(m, (we + R - 1) / R);
   if (cs > 0) cs--;
-  int64_t *cidx = (int64_t *)c->scratch("coarse_idx", pairs * sizeof(int64_t));
-  cf *cval = (cf *)c->scratch("coarse_val", pairs * sizeof(cf));
-  int2 *list = (int2 *)c->scratch("refine_list", pairs * sizeof(int2));
-  int *count = hier_counter(c, c->hier_parity);
-  int *other = hier_counter(c, c->hier_parity ^ 1);
+  int64_t *cidx;
+  cf *cval;
+  int2 *list;
+  int *count, *other;
+  if (hs) {
+    // caller-supplied buffers: this call's counter starts at zero here, and
+    // the screen's clearing of "the next call's counter" lands on a spare word
+    cidx = hs->cidx;
+    cval = hs->cval;
+    list = hs->list;
+    count = hs->count;
+    other = hs->spare;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, c->stream, count, 1);
+    if (!hip_ok(hipGetLastError(), "zero_ints_kernel")) return -1;
+  } else {
+    cidx = (int64_t *)c->scratch("coarse_idx", pairs * sizeof(int64_t));
+    cval = (cf *)c->scratch("coarse_val", pairs * sizeof(cf));
+    list = (int2 *)c->scratch("refine_list", pairs * sizeof(int2));
+    count = hier_counter(c, c->hier_parity);
+    other = hier_counter(c, c->hier_parity ^ 1);
+  }
   if (!cidx || !cval || !list || !count || !other) return -1;
 
   const cf *drows = data_rows ? data_rows : c->data + (size_t)d0 * n;
@@ -202,8 +226,9 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
     if (!hip_ok(hipGetLastError(), "gate_kernel")) return -1;
   }
   // the screen has cleared the other counter: the next call uses it (a
-  // capture flips the parity when its graph runs)
-  if (!c->capturing) c->hier_parity ^= 1;
+  // capture flips the parity when its graph runs; caller-supplied counters
+  // leave the context's pair untouched)
+  if (!c->capturing && !hs) c->hier_parity ^= 1;
 
   FlatRun rr;
   rr.data = drows;
@@ -446,4 +471,67 @@ extern "C" long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, siz
     total += cnt;
   }
   return total;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free hierarchical run.  The per-call buffers (coarse outputs,
+// survivor list, counter) live in one file-scope device array, `scratch`, of
+// fixed size; each device holds its own copy, whose address is looked up once
+// per device with hipGetSymbolAddress (the array's name in host code is not a
+// device address).  Every buffer the context itself needs (coarse bank,
+// twiddle and MFMA tables) must already exist: with Ctx::no_alloc set, any
+// attempt to create one fails the call instead of allocating.
+static constexpr size_t kNoAllocMaxPairs = (size_t)1 << 18;
+static constexpr size_t kNoAllocHead = 256;  // counter words, one 256-byte line
+static constexpr size_t kNoAllocBytes =
+    kNoAllocHead + kNoAllocMaxPairs * (sizeof(int64_t) + sizeof(mfh::cf) + sizeof(int2));  // 6 MiB + 256 B
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+std::mutex g_scratch_mutex;
+std::map<int, unsigned char *> g_scratch_ptr;  // device -> address of its copy of `scratch`
+
+// The calling thread's current device must be `dev`.
+unsigned char *device_scratch(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void *p = nullptr;
+  if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress")) return nullptr;
+  if (!p) {
+    fail("hipGetSymbolAddress returned a null address for the scratch array");
+    return nullptr;
+  }
+  g_scratch_ptr[dev] = (unsigned char *)p;
+  return (unsigned char *)p;
+}
+
+}  // namespace
+
+extern "C" int mfh_hier_peaks_noalloc(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize,
+                                      float threshold, size_t ws, size_t we, float coarse_thr, int64_t *dev_idx,
+                                      void *dev_val_c64) {
+  if (!ctx) return fail("null context");
+  if (!dev_idx || !dev_val_c64) return fail("null output pointer");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  if (nd < 1 || nt < 1) return fail("data/templates sub-range out of bounds");
+  const size_t pairs = (size_t)nd * (size_t)nt;
+  if (pairs > kNoAllocMaxPairs)
+    return fail("%zu pairs exceed the allocation-free limit of %zu pairs per call", pairs, kNoAllocMaxPairs);
+  if (!hip_ok(hipSetDevice(c->dev), "hipSetDevice")) return -1;
+  unsigned char *base = device_scratch(c->dev);
+  if (!base) return -1;
+  mfh::HierScratch hs;
+  hs.count = (int *)base;
+  hs.spare = (int *)base + 1;
+  hs.cidx = (int64_t *)(base + kNoAllocHead);
+  hs.cval = (mfh::cf *)(base + kNoAllocHead + pairs * sizeof(int64_t));
+  hs.list = (int2 *)(base + kNoAllocHead + pairs * (sizeof(int64_t) + sizeof(mfh::cf)));
+  c->no_alloc = true;
+  const int rc = mfh::hier_run(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, dev_idx,
+                               (mfh::cf *)dev_val_c64, nullptr, nullptr, &hs);
+  c->no_alloc = false;
+  return rc;
 }

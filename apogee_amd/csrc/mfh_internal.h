@@ -75,6 +75,9 @@ struct Ctx {
   };
   bool prof_on = false;
   bool capturing = false;  // a stream capture is recording launches
+  // set for the duration of an allocation-free call: scratch() then fails
+  // instead of allocating, and no timing events are taken
+  bool no_alloc = false;
   std::vector<Ev> prof_events;
   std::vector<hipEvent_t> event_pool;
   hipEvent_t take_event();
@@ -170,7 +173,18 @@ int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, int t0, in
 // Hierarchical run over data rows [d0, d0+nd) of the context, or over
 // `data_rows` (stride n) when not null.
 int *hier_counter(Ctx *c, int parity);
+// Per-call buffers of a hierarchical run supplied by the caller instead of
+// the context: coarse outputs, the survivor list and its counter (cleared by
+// the run), and a spare counter word the screen may clear.
+struct HierScratch {
+  int64_t *cidx = nullptr;
+  cf *cval = nullptr;
+  int2 *list = nullptr;
+  int *count = nullptr;
+  int *spare = nullptr;
+};
 int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,
-             float coarse_thr, int64_t *idx, cf *val, int *count_out, const cf *data_rows);
+             float coarse_thr, int64_t *idx, cf *val, int *count_out, const cf *data_rows,
+             const HierScratch *hs = nullptr);
 
 }  // namespace mfh

@@ -173,6 +173,34 @@ long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
                     float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
                     int64_t *n_refined);
 
+/* mfh_hier_peaks with device outputs, queued on the stream; if host_refined
+ * is not NULL it receives the survivor count once the stream reaches that
+ * point.
+ * Replaces: the one-command-buffer hierarchical recording of
+ * Context._make_hier (python/matchedfilter/_vkcompute.py:650-850). */
+int mfh_hier_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
+                         size_t binsize, float threshold, size_t ws, size_t we,
+                         float coarse_thr, int64_t *dev_idx, void *dev_val_c64,
+                         int32_t *host_refined);
+
+/* mfh_hier_peaks_async that allocates nothing: no hipMalloc, hipFree or
+ * allocating library call, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it).  The
+ * per-call buffers (coarse outputs, survivor list and counter) live in one
+ * fixed device array of the library, 6 MiB + 256 B per GPU, held until the
+ * process ends; at most 262144 (1 << 18) pairs per call, more is rejected
+ * with an error.  Every buffer of the context itself (coarse bank, twiddle
+ * and matrix-core tables for this band and precision) must already exist:
+ * one mfh_hier_peaks or mfh_hier_peaks_async call on the context builds
+ * them; otherwise the call fails with an error and launches nothing.  All
+ * calls on one GPU share the array: calls that overlap on one GPU (other
+ * streams, contexts or threads) race, and must be ordered by the caller.
+ * The counter is cleared by a kernel at the start of each call (no memset).
+ * Replaces: nothing in the reference, which has no allocation-free entry. */
+int mfh_hier_peaks_noalloc(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
+                           size_t binsize, float threshold, size_t ws, size_t we,
+                           float coarse_thr, int64_t *dev_idx, void *dev_val_c64);
+
 /* Coarse screen precision: 0 = FP32 (the band-point flat filter, exact
  * reference semantics), 1 = fp16 or 2 = bf16 matrix cores (band 512 only;
  * FP32 accumulation), whose statistic is widened by (1 + delta) at the gate
@@ -210,15 +238,6 @@ int mfh_probe_valu(int device, int op, int chains, int blocks, int iters, double
  * multiple of 16), device to device; writes the second launch's duration.
  * Replaces: the COPY probe of tools/gpu_roofline.py:236-243. */
 int mfh_probe_copy(int device, size_t bytes, double *ms_out);
-
-/* Same with device outputs, queued on the stream; if host_refined is not
- * NULL it receives the survivor count once the stream reaches that point.
- * Replaces: the one-command-buffer hierarchical recording of
- * Context._make_hier (python/matchedfilter/_vkcompute.py:650-850). */
-int mfh_hier_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
-                         size_t binsize, float threshold, size_t ws, size_t we,
-                         float coarse_thr, int64_t *dev_idx, void *dev_val_c64,
-                         int32_t *host_refined);
 
 /* ---- time series --------------------------------------------------------- */
 

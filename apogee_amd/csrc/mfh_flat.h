@@ -357,8 +357,14 @@ __device__ __forceinline__ void stockham_load(cf *v, int t, const cf *xch) {
 // barrier that protects the buffer from the next store sits just before that
 // store, after the pass's arithmetic, so waves do not idle between load and
 // compute.  The caller must barrier before the next call's first store.
-template <int N>
-__device__ __forceinline__ void inverse_fft(cf *v, int t, cf *xch, const cf *tw) {
+//
+// With HOISTED, `wfirst` holds the twiddles of the first twiddled radix-16
+// pass (first_twiddles below), derived once by the caller: they depend on
+// the thread only, so a kernel that runs many pairs per thread derives them
+// once instead of per pair.  The values, and so the results, are bit for bit
+// those of the per-pair derivation.
+template <int N, bool HOISTED = false>
+__device__ __forceinline__ void inverse_fft(cf *v, int t, cf *xch, const cf *tw, const cf *wfirst = nullptr) {
   using S = FlatShape<N>;
   using L = TwiddleLayout<N>;
   constexpr int F = S::FIRST;
@@ -367,18 +373,18 @@ __device__ __forceinline__ void inverse_fft(cf *v, int t, cf *xch, const cf *tw)
   if constexpr (F > 1) {
     stockham_pass<N, F, 1>(v, w);
     stockham_store<N, F, 1>(v, t, xch);
-    load_twiddles16<N, NS0>(w, t, tw + L::offset(0));
+    if constexpr (!HOISTED) load_twiddles16<N, NS0>(w, t, tw + L::offset(0));
     __syncthreads();
     stockham_load<N>(v, t, xch);
   }
-  stockham_pass<N, 16, NS0>(v, w);
+  stockham_pass<N, 16, NS0>(v, (HOISTED && F > 1) ? wfirst : w);
   if constexpr (S::NPASS16 >= 2) {
     if constexpr (F > 1) __syncthreads();
     stockham_store<N, 16, NS0>(v, t, xch);
-    load_twiddles16<N, NS0 * 16>(w, t, tw + L::offset(1));
+    if constexpr (!(HOISTED && F == 1)) load_twiddles16<N, NS0 * 16>(w, t, tw + L::offset(1));
     __syncthreads();
     stockham_load<N>(v, t, xch);
-    stockham_pass<N, 16, NS0 * 16>(v, w);
+    stockham_pass<N, 16, NS0 * 16>(v, (HOISTED && F == 1) ? wfirst : w);
   }
   if constexpr (S::NPASS16 >= 3) {
     __syncthreads();
@@ -389,6 +395,21 @@ __device__ __forceinline__ void inverse_fft(cf *v, int t, cf *xch, const cf *tw)
     stockham_pass<N, 16, NS0 * 256>(v, w);
   }
   static_assert(S::NPASS16 >= 1 && S::NPASS16 <= 3, "N range");
+}
+
+// The twiddles of the first twiddled radix-16 pass of a length-N transform
+// for thread t (the pass after the small-radix one, or the second radix-16
+// pass when N is a power of 16), as inverse_fft<N, true> takes them.
+template <int N>
+__device__ __forceinline__ void first_twiddles(cf *w, int t, const cf *tw) {
+  using S = FlatShape<N>;
+  using L = TwiddleLayout<N>;
+  if constexpr (S::FIRST > 1) {
+    load_twiddles16<N, L::NS0>(w, t, tw + L::offset(0));
+  } else {
+    static_assert(S::NPASS16 >= 2, "a power of 16 has a second radix-16 pass");
+    load_twiddles16<N, 16>(w, t, tw + L::offset(1));
+  }
 }
 
 // Bin-table modes of the flat kernel.
@@ -420,15 +441,13 @@ flat_kernel(FlatArgs a) {
   bool valid = tt < a.nt;
   int dbeg = blockIdx.y * a.dpw;
   int dend = min(a.nd, dbeg + a.dpw);
-  cf T[16];
   if constexpr (LIST) {
     dbeg = blockIdx.x * S::SLOTS;
     dend = *a.list_count;
-  } else {
-    const cf *tp = a.tmpl + (long long)(valid ? tt : 0) * a.tmpl_stride;
-#pragma unroll
-    for (int u = 0; u < 16; u++) T[u] = valid ? tp[t0 + u * TPT] : cf{0.f, 0.f};
   }
+  // the first twiddled pass's twiddles, once for every pair this thread runs
+  cf wfirst[15];
+  first_twiddles<N>(wfirst, t0, a.twiddle);
   if (a.zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *a.zero = 0;
   if constexpr (LIST) {
     if (dbeg >= dend) return;  // workgroup-uniform: the list is shorter than this slot range
@@ -468,13 +487,24 @@ flat_kernel(FlatArgs a) {
         for (int u = 0; u < 16; u++) v[u] = cmul_conj(dp[t + u * TPT], tp[t + u * TPT]);
       }
     } else {
+      // the slot's template is read again for every data row of the tile
+      // (from the caches after the first): holding it in registers would
+      // cost the 32 VGPRs the hoisted twiddles use.  A slot past the bank
+      // (valid false) runs template 0 and writes nothing.
       const cf *dp = a.data + (long long)d * a.data_stride;  // uniform: d is the tile's row
-      cf dv[16];
+      const cf *tp = a.tmpl + (long long)(valid ? tt : 0) * a.tmpl_stride;
+      cf dv[16], tv[16];
       load_row_uniform<N>(dv, dp, t);
+      if constexpr (TPT >= 64) {
+        load_row_uniform<N>(tv, uniform_ptr(tp), t);
+      } else {
 #pragma unroll
-      for (int u = 0; u < 16; u++) v[u] = cmul_conj(dv[u], T[u]);
+        for (int u = 0; u < 16; u++) tv[u] = tp[t + u * TPT];
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) v[u] = cmul_conj(dv[u], tv[u]);
     }
-    inverse_fft<N>(v, t, xch, a.twiddle);
+    inverse_fft<N, true>(v, t, xch, a.twiddle, wfirst);
 
     if constexpr (MODE == kOneBin) {
       // ---- one bin: a wave key, one LDS max; the winning lag's owner writes ----

@@ -157,6 +157,21 @@ def probes():
     return out
 
 
+def settle(step, sync, seconds=0.5):
+    """Back-to-back steps for `seconds` before anything is timed.  The GPU's
+    clocks rise over the first tens of milliseconds under load: on one
+    MI355X the flat step averaged 118 us over 50 steps from idle against
+    95 us over 400, and the first launches of a run took 106-112 us against
+    93-94 us later (rocprofv3 kernel trace).  The reference warms its
+    published timings for 0.5 s for the same reason
+    (tools/teaser_figure.py:46-54)."""
+    until = time.perf_counter() + seconds
+    while time.perf_counter() < until:
+        for _ in range(10):
+            step()
+        sync()
+
+
 def public_run_time(fn, reps=7, block_s=0.05):
     """The reference's timing of its published figures: warm calls for 0.5 s,
     then the median over `reps` blocks of >= `block_s` of back-to-back calls."""
@@ -189,9 +204,7 @@ def timed_calls(fn, ctx, steps):
     """Wall time per call with profiling off (the best of three runs of
     `steps` back-to-back calls, after 50 untimed ones: the host's launch path
     settles over the first few dozen calls), then per-stage event times."""
-    for _ in range(50):
-        fn()
-    ctx.sync()
+    settle(fn, ctx.sync)
     dt = float("inf")
     for _ in range(3):
         t0 = time.perf_counter()
@@ -361,9 +374,7 @@ def large_n_leg(args, n=65536, nd=4, nt=128, binsize=1024, threshold=5.5):
     def step():
         ctx.peaks_async(0, nd, 0, nt, binsize, threshold, 0, n, idx.data_ptr(), val.data_ptr())
 
-    for _ in range(5):
-        step()
-    ctx.sync()
+    settle(step, ctx.sync)
     steps = max(10, args.steps // 4)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -407,8 +418,7 @@ def series_leg(args):
         f = mf.MatchedFilter(n, 1, nt, device="gpu:0")
         f.set_templates(T)
         ser = series.to(torch.complex64)
-        for _ in range(3):
-            f.run_series(ser, starts, ws, we, binsize=bs, threshold=5.5)
+        settle(lambda: f.run_series(ser, starts, ws, we, binsize=bs, threshold=5.5), torch.cuda.synchronize)
         steps = max(5, args.steps // 20)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -435,7 +445,7 @@ def uneven_leg(args, nreq=4000):
     val = torch.empty(q.total, dtype=torch.complex64, device="cuda")
     for _ in range(3):                      # eager, capture, first replay
         q.run_async(idx, val)
-    q.sync()
+    settle(lambda: q.run_async(idx, val), q.sync)
     steps = max(10, args.steps // 4)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -504,9 +514,7 @@ def fit_leg(args):
         val, _ = pf(D, T, binsize=bs)
         (-(val.abs() ** 2).sum()).backward()
 
-    for _ in range(3):
-        fwd_bwd()
-    torch.cuda.synchronize()
+    settle(fwd_bwd, torch.cuda.synchronize)
     t0 = time.perf_counter()
     for _ in range(steps):
         val, _ = pf(D, T, binsize=bs)
@@ -571,6 +579,7 @@ def main():
     def step():
         ctx.peaks_async(0, ND, 0, NT, N, THRESHOLD, 0, N, idx.data_ptr(), val.data_ptr())
 
+    settle(step, ctx.sync)
     for _ in range(args.warmup):
         step()
     ctx.sync()

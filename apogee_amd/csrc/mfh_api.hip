@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "mfhip.h"
@@ -35,6 +37,18 @@ bool hip_ok(hipError_t e, const char *what) {
   if (e == hipSuccess) return true;
   fail("%s: %s", what, hipGetErrorString(e));
   return false;
+}
+
+bool lds_limit(const void *fn, int dev, size_t bytes) {
+  static std::mutex mu;
+  static std::map<std::pair<const void *, int>, size_t> done;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t &have = done[{fn, dev}];
+  if (bytes <= have) return true;
+  if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes), "hipFuncSetAttribute"))
+    return false;
+  have = bytes;
+  return true;
 }
 
 Ctx::~Ctx() {

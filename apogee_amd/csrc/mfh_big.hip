@@ -286,17 +286,9 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
   a.inv_bs = 1.0f / (float)r.binsize;
   const size_t nb_total = (r.we - r.ws + r.binsize - 1) / r.binsize;
   const size_t maxb = rows_max_bins<N1>();
-  static bool attr_set = false;
-  if (!attr_set) {
-    if (!hip_ok(hipFuncSetAttribute((const void *)big_cols_kernel<N1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds_cols),
-                "hipFuncSetAttribute") ||
-        !hip_ok(hipFuncSetAttribute((const void *)big_rows_kernel<N1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)kLds),
-                "hipFuncSetAttribute"))
-      return -1;
-    attr_set = true;
-  }
+  if (!lds_limit((const void *)big_cols_kernel<N1>, c->dev, lds_cols) ||
+      !lds_limit((const void *)big_rows_kernel<N1>, c->dev, kLds))
+    return -1;
   // rows workgroups per pair: enough for about 1024 workgroups per launch (at
   // least 8, at most one per row group), as long as the per-part tables stay
   // within 64 MiB

@@ -32,13 +32,7 @@ template <int N, int MODE, bool LIST>
 int launch_mode(Ctx *c, const FlatArgs &a, dim3 grid, size_t lds) {
   using S = FlatShape<N>;
   const void *fn = (const void *)flat_kernel<N, MODE, LIST>;
-  static size_t lds_set = 0;
-  if (lds > lds_set) {
-    if (!hip_ok(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                "hipFuncSetAttribute"))
-      return -1;
-    lds_set = lds;
-  }
+  if (!lds_limit(fn, c->dev, lds)) return -1;
   hipLaunchKernelGGL((flat_kernel<N, MODE, LIST>), grid, dim3(S::WG), lds, c->stream, a);
   return hip_ok(hipGetLastError(), "flat_kernel launch") ? 0 : -1;
 }

@@ -13,6 +13,10 @@ namespace mfh {
 
 int fail(const char *fmt, ...);
 bool hip_ok(hipError_t e, const char *what);
+// Raise kernel `fn`'s dynamic LDS limit to at least `bytes` on the current
+// device `dev`, once per (kernel, device) and size: the attribute belongs to
+// a device, so a second GPU in the process gets its own call.
+bool lds_limit(const void *fn, int dev, size_t bytes);
 
 struct Ctx {
   int dev = -1;

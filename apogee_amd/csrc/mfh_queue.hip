@@ -273,15 +273,9 @@ extern "C" mfh_queue *mfh_queue_create(int device) {
     return nullptr;
   }
   q->cus = prop.multiProcessorCount;
-  static bool attr_set = false;
-  if (!attr_set) {
-    if (!hip_ok(hipFuncSetAttribute((const void *)mfh::queue_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)mfh::kQLds),
-                "hipFuncSetAttribute")) {
-      delete q;
-      return nullptr;
-    }
-    attr_set = true;
+  if (!mfh::lds_limit((const void *)mfh::queue_kernel, q->c.dev, mfh::kQLds)) {
+    delete q;
+    return nullptr;
   }
   return reinterpret_cast<mfh_queue *>(q);
 }

@@ -58,14 +58,7 @@ int launch_fwd(Ctx *c, const cf *series, size_t nseries, const long long *starts
   const cf *tw = c->twiddles(N);
   if (!tw) return -1;
   const size_t lds = (size_t)S::SLOTS * S::XSTRIDE * sizeof(cf);
-  static bool lds_set = false;
-  if (!lds_set) {
-    if (!hip_ok(hipFuncSetAttribute((const void *)series_forward_kernel<N>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                "hipFuncSetAttribute"))
-      return -1;
-    lds_set = true;
-  }
+  if (!lds_limit((const void *)series_forward_kernel<N>, c->dev, lds)) return -1;
   Ctx::ProfScope ps(c, "series_forward");
   hipLaunchKernelGGL(series_forward_kernel<N>, dim3((unsigned)((nblocks + S::SLOTS - 1) / S::SLOTS)), dim3(S::WG),
                      lds, c->stream, series, (long long)nseries, starts, nblocks, tw, out);

@@ -103,9 +103,15 @@ def test_series_matches_block_by_block_runs(gpu_device):
         f.set_data((np.fft.fft(blk) / n).astype(np.complex64)[None, :])
         want = f.run(binsize=n, threshold=0.0, window=(int(ws[b]), int(we[b])))
         same = got["index"][b] == want["index"][0]
-        assert same.mean() >= 0.75          # FP32 forward transforms differ only at near-ties
         np.testing.assert_allclose(np.abs(got["value"][b][same]), np.abs(want["value"][0][same]),
                                    rtol=2e-6 * 12)
+        # the two FP32 forward transforms (device, NumPy) may only swap lags
+        # whose FP64 magnitudes are within the near-tie margin of each other
+        absz = np.abs(O.correlate(O.series_spectra(series, [int(s)], n)[:, None, :], T[None]))[0]
+        for t in np.nonzero(~same)[0]:
+            a, w = int(got["index"][b][t, 0]), int(want["index"][0][t, 0])
+            top = absz[t, int(ws[b]):int(we[b])].max()
+            assert abs(absz[t, a] - absz[t, w]) <= 4 * 2e-6 * 12 * top, (b, t, a, w)
 
 
 def test_hierarchical_series_is_identical_or_absent(gpu_device):

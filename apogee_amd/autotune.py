@@ -11,7 +11,11 @@ Tunables (each a key of the library's tuning interface):
   each wave of the matrix-core coarse screen takes (more of either: the
   per-wave set-up is shared by more pairs; fewer: more waves in flight);
 * ``queue`` -- ``grid_per_cu``: persistent workgroups per CU of the
-  mixed-length refine queue.
+  mixed-length refine queue;
+* ``band`` -- not a launch knob but the measured costs the hierarchical
+  filter chooses its coarse band from: per length, the cost of the screen
+  alone and of refining every pair, per band, at two batch sizes
+  (HierarchicalFilter's automatic configuration).
 
 ``python -m apogee_amd.autotune --out apogee_amd/tuning_gfx950.json`` times
 every candidate on the device with HIP events (median of repeated launches,
@@ -45,6 +49,11 @@ FLAT_SHAPES = [
 HIER_SHAPES = [(4096, 16, 1024, 4096, 512)]
 COARSE_SHAPES = [(4096, 16, 1024, 512), (4096, 64, 256, 512), (4096, 4, 4096, 512)]
 QUEUE_WORKLOADS = [("uneven-4000", 4000)]
+# hierarchical band costs: (n, ndata, ntemplates) of the full batch, timed
+# also at a small batch (4 x 64 pairs) so cost can be interpolated in pairs
+BAND_SHAPES = [(256, 16, 256), (512, 16, 256), (1024, 16, 1024), (2048, 16, 512), (4096, 16, 1024),
+               (8192, 16, 256), (16384, 16, 256), (32768, 4, 128), (65536, 4, 128)]
+BAND_MAX = 16384      # the largest band the one-pass coarse screen runs
 
 _table = None
 
@@ -205,6 +214,78 @@ def uneven_workload(nreq, seed, device="cuda"):
     return pool, dict(n=n, data_offset=doff, template_offset=doff + n, candidates=cand, binsize=n // 8)
 
 
+def band_entry(n):
+    """The measured band-cost entry of length n, or None."""
+    for e in _load().get("entries", []):
+        if e.get("op") == "band" and e.get("n") == n:
+            return e
+    return None
+
+
+def _stage_ms(ctx, run, reps):
+    """Median over reps of the summed event time of one hierarchical call."""
+    import statistics
+    run()
+    ctx.sync()
+    times = []
+    for _ in range(reps):
+        ctx.prof_enable(True)
+        run()
+        ctx.sync()
+        times.append(sum(ctx.prof_read(k)[0] for k in ("coarse", "gate", "refine")))
+    ctx.prof_enable(False)
+    return statistics.median(times)
+
+
+def _measure_band(device, n, nd, nt, reps, rng):
+    """Hierarchical call costs per coarse band at length n: the screen alone
+    (a gate nothing passes: coarse screen, gate and an empty refine launch)
+    and with every pair refined, each at a small (4 x 64) and the full
+    (nd x nt) batch, FP32 screen for every band and the fp16 matrix-core
+    screen at band 512.  Unit-norm white templates, complex N(0,1) data."""
+    import numpy as np
+    import torch
+    from . import _hip
+    ctx = _hip.Context(device, n, nd, nt)
+    ctx.set_rows("data", 0, (rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))).astype(np.complex64))
+    T = rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))
+    ctx.set_rows("template", 0, (T / np.linalg.norm(T, axis=1, keepdims=True)).astype(np.complex64))
+    idx = torch.empty(nd * nt, dtype=torch.int64, device="cuda:%d" % device)
+    val = torch.empty(nd * nt, dtype=torch.complex64, device="cuda:%d" % device)
+    shapes = [(min(nd, 4), min(nt, 64)), (nd, nt)]
+
+    def hier(snd, snt, gate):
+        return lambda: ctx.hier_peaks_async(0, snd, 0, snt, n, 0.0, 0, n, gate, idx.data_ptr(), val.data_ptr())
+
+    flat = []
+    for snd, snt in shapes:
+        run = lambda: ctx.peaks_async(0, snd, 0, snt, n, 0.0, 0, n, idx.data_ptr(), val.data_ptr())
+        run()
+        ctx.sync()
+        flat.append(round(_median_ms(run, reps, ctx, "flat"), 5))
+    bands = {}
+    b = 64
+    while b < n and b <= BAND_MAX:
+        ctx.hier_set_band(b, -1.0)
+        ctx.hier_set_coarse_precision(0, 0.0)
+        row = {"screen_ms": [], "refine_ms_all": []}
+        for snd, snt in shapes:
+            screen = _stage_ms(ctx, hier(snd, snt, 3.0e38), reps)
+            every = _stage_ms(ctx, hier(snd, snt, 0.0), reps)
+            row["screen_ms"].append(round(screen, 5))
+            row["refine_ms_all"].append(round(max(every - screen, 0.0), 5))
+        if b == 512:
+            ctx.hier_set_coarse_precision(1, 2e-3)
+            row["screen_ms_fp16"] = [round(_stage_ms(ctx, hier(snd, snt, 3.0e38), reps), 5)
+                                     for snd, snt in shapes]
+            ctx.hier_set_coarse_precision(0, 0.0)
+        bands[str(b)] = row
+        b *= 2
+    ctx.destroy()
+    return {"op": "band", "n": n, "ndata": nd, "ntemplates": nt,
+            "pairs": [snd * snt for snd, snt in shapes], "flat_ms": flat, "bands": bands}
+
+
 def _tune_queue(device, name, nreq, reps):
     from .refine import RefineQueue
     import torch
@@ -253,6 +334,10 @@ def main(argv=None):
         entries.append(e)
     for n, nd, nt, band in COARSE_SHAPES:
         e = _tune_coarse(args.device, n, nd, nt, band, args.reps, rng)
+        print(json.dumps(e), flush=True)
+        entries.append(e)
+    for n, nd, nt in BAND_SHAPES:
+        e = _measure_band(args.device, n, nd, nt, args.reps, rng)
         print(json.dumps(e), flush=True)
         entries.append(e)
     for wname, nreq in QUEUE_WORKLOADS:

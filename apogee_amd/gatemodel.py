@@ -166,3 +166,25 @@ def dismissal(power, n, band, snr, gate, fd_hint=1e-3):
     if kept is None or not len(kept):
         return None
     return float(np.searchsorted(kept, float(gate)) / len(kept))
+
+
+def noise_exceedance(power, n, band, gate):
+    """Modelled fraction of noise-only pairs whose coarse maximum reaches
+    ``gate``: the refine rate the gate costs on pure noise.
+
+    The coarse series has band lags whose noise is correlated through the
+    in-band profile; it behaves like ``m`` independent lags, with m the
+    profile's effective number of in-band bins B = (sum q)^2 / sum q^2 (at
+    most band), each with |z|^2 exponential of mean 2 (the unit-variance
+    convention above): P = 1 - (1 - exp(-gate^2 / 2))^m.  An approximation
+    for ranking bands, not a calibration: the gate itself comes from
+    ``gate_for``.
+    """
+    _validate(n, band, 1.0, 0.5)
+    q = np.asarray(power, dtype=np.float64)[:band]
+    q = np.where(q > 0, q, 0.0)
+    if q.sum() <= 0:
+        return 0.0
+    m = min(float(band), max(1.0, q.sum() ** 2 / np.square(q).sum()))
+    tail = np.exp(-0.5 * float(gate) ** 2)
+    return float(-np.expm1(m * np.log1p(-tail))) if tail < 1 else 1.0

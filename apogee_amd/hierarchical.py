@@ -12,10 +12,17 @@ report -1 / 0.  Every reported peak is the flat filter's peak; only omissions
 are possible.
 
 Configuration: an explicit ``band`` (and optionally ``set_coarse_threshold``)
-is used as given.  Otherwise the band comes from this build's gfx950 band
-table and the coarse gate from the reference profile through
-``apogee_amd.gatemodel.gate_for`` at (snr, fd), which needs
-``set_reference``.
+is used as given.  Otherwise (``set_reference`` required) the band is the
+cheapest on this device by measured cost, as the reference ranks its
+measured cost rows (``choose_config``, python/matchedfilter/__init__.py:
+996-1052): for every band whose modelled gate resolves the budget
+(``gatemodel.gate_for`` at (snr, fd)), the expected cost of a call of this
+filter's ndata x ntemplates pairs is the measured screen cost plus the
+modelled noise refine rate at that gate (``gatemodel.noise_exceedance``)
+times the measured cost of refining every pair, both interpolated in the
+pair count from the gfx950 table (``autotune``, op ``band``); the smallest
+cost wins.  Lengths the table lacks fall back to the band n/8 (at most
+16384).
 """
 import numpy as np
 
@@ -29,11 +36,45 @@ from . import MatchedFilter, _host_view
 #: defaults are about 4x that.
 _COARSE_PRECISIONS = {"fp32": (0, 0.0), "fp16": (1, 2e-3), "bf16": (2, 1.6e-2)}
 
-#: Coarse band per transform length on gfx950 (power of two, 64 <= band < n).
-#: At n = 4096 the 512-point band is what the benchmark configuration pins.
-_GFX950_BAND = {1024: 128, 2048: 256, 4096: 512, 8192: 1024, 16384: 2048,
-                32768: 4096, 65536: 8192, 131072: 16384, 262144: 32768, 524288: 65536,
-                1048576: 131072}
+#: The largest band the one-pass coarse screen runs.
+_BAND_MAX = 16384
+
+
+def _in_pairs(values, at, pairs):
+    """Cost at ``pairs`` from costs measured at two batch sizes, linear in
+    the pair count (a fixed launch part plus a per-pair part)."""
+    (p0, p1), (c0, c1) = at, values
+    if p1 == p0:
+        return float(c1)
+    slope = (c1 - c0) / (p1 - p0)
+    return max(float(c0 + slope * (pairs - p0)), 0.0)
+
+
+def rank_bands(power, n, snr, fd, pairs, precision="fp32", entry=None):
+    """Candidate bands of length n, cheapest first, from the measured cost
+    entry (default: the gfx950 table's): a list of dicts with band, gate,
+    modelled noise refine rate and expected milliseconds per call.  Bands
+    whose gate cannot resolve fd are left out."""
+    from . import autotune
+    from .gatemodel import gate_for, noise_exceedance
+    entry = autotune.band_entry(n) if entry is None else entry
+    if entry is None:
+        return []
+    out = []
+    for key, row in entry["bands"].items():
+        band = int(key)
+        if band >= n or band > _BAND_MAX:
+            continue
+        gate = gate_for(power, n, band, snr, fd)
+        if gate is None:
+            continue
+        rate = noise_exceedance(power, n, band, gate)
+        screen = row.get("screen_ms_fp16") if (precision != "fp32" and band == 512) else None
+        screen = row["screen_ms"] if screen is None else screen
+        cost = (_in_pairs(screen, entry["pairs"], pairs)
+                + rate * _in_pairs(row["refine_ms_all"], entry["pairs"], pairs))
+        out.append({"band": band, "gate": gate, "noise_refine_rate": rate, "ms": cost})
+    return sorted(out, key=lambda c: (c["ms"], c["band"]))
 
 
 class HierarchicalFilter(MatchedFilter):
@@ -69,6 +110,7 @@ class HierarchicalFilter(MatchedFilter):
         self._trig = 0
         self._prec = ("fp32", 0.0)
         self._prec_set = None
+        self._auto = None
 
     def _apply_tuning(self):
         from . import autotune
@@ -168,13 +210,20 @@ class HierarchicalFilter(MatchedFilter):
             return self._pinned[0]
         if self._ref is None:
             raise ValueError("set_reference is required for automatic configuration selection")
-        band = _GFX950_BAND.get(self.n)
-        if band is None:
-            raise ValueError("no gfx950 band for n=%d: pass band= explicitly" % self.n)
+        key = (self.snr, self.fd, self._fs_snr, self._prec[0], self._ref.tobytes())
+        if self._auto is not None and self._auto[0] == key:
+            return self._auto[1]
+        ranked = rank_bands(self._ref, self.n, self._fs_snr or self.snr, self.fd,
+                            self.ndata * self.ntemplates, self._prec[0])
+        if ranked:
+            band = ranked[0]["band"]
+        else:
+            band = min(self.n // 8, _BAND_MAX)
+        self._auto = (key, band)
         return band
 
     def _calibration(self):
-        key = (self.snr, self.fd, self._fs_snr, self._pinned, self._cal_thr,
+        key = (self.snr, self.fd, self._fs_snr, self._pinned, self._cal_thr, self._prec[0],
                None if self._ref is None else self._ref.tobytes())
         if self._cal is not None and self._cal[0] == key:
             return self._cal[1]
@@ -232,7 +281,8 @@ class HierarchicalFilter(MatchedFilter):
 
     @property
     def cost_table(self):
-        """Which band table chose the band: None when pinned."""
+        """Which cost table chose the band: None when pinned, "gfx950" for
+        the measured table (or its n/8 fallback for lengths it lacks)."""
         return None if self._pinned is not None else "gfx950"
 
     @property

@@ -153,7 +153,9 @@ def test_hierarchical_validation_and_config(fake):
     with pytest.raises(ValueError):
         h.config                           # automatic band needs a reference
     h.set_reference(inspiral_power(4096))
-    assert h.config == (512, 8) and h.cost_table == "gfx950"
+    band, taps = h.config
+    assert taps == 8 and h.cost_table == "gfx950"
+    assert 64 <= band < 4096 and band & (band - 1) == 0
     with pytest.raises(ValueError):
         h.set_reference(np.ones(10))
     with pytest.raises(ValueError):
@@ -335,3 +337,72 @@ def test_autotune_table_lookup(tmp_path, monkeypatch):
     monkeypatch.setattr(autotune, "TABLE_PATH", str(tmp_path / "missing.json"))
     monkeypatch.setattr(autotune, "_table", None)
     assert autotune.lookup("flat", 4096, 16, 1024) == {}
+
+
+def synthetic_band_entry(n=4096, screen=None, refine=None):
+    """A band-cost entry in the table's format: screen and all-refined
+    milliseconds per band at 256 and 16384 pairs."""
+    screen = screen or {64: (4.0, 30.0), 128: (4.0, 12.0), 256: (4.0, 8.0), 512: (4.0, 10.0),
+                        1024: (4.0, 14.0), 2048: (4.0, 24.0)}
+    refine = refine or (2.0, 100.0)
+    return {"op": "band", "n": n, "ndata": 16, "ntemplates": 1024, "pairs": [256, 16384],
+            "flat_ms": [2.0, 100.0],
+            "bands": {str(b): {"screen_ms": list(c), "refine_ms_all": list(refine),
+                               **({"screen_ms_fp16": [3.0, 5.0]} if b == 512 else {})}
+                      for b, c in screen.items()}}
+
+
+def test_band_ranking_weighs_screen_cost_against_the_noise_refine_rate():
+    """The cheapest measured band wins; a wide refine bill pushes the choice
+    to bands whose higher gate refines less; the matrix-core screen's cost
+    counts only with an fp16/bf16 screen; pair count interpolates."""
+    from apogee_amd.hierarchical import rank_bands
+    from apogee_amd import gatemodel as G
+    p = inspiral_power(4096)
+    ranked = rank_bands(p, 4096, 5.5, 1e-2, 16384, entry=synthetic_band_entry())
+    assert ranked and ranked == sorted(ranked, key=lambda c: c["ms"])
+    for c in ranked:
+        assert c["gate"] == G.gate_for(p, 4096, c["band"], 5.5, 1e-2)
+        assert 0.0 <= c["noise_refine_rate"] <= 1.0
+    # refine rate falls as the band (and its gate) grows
+    rates = {c["band"]: c["noise_refine_rate"] for c in ranked}
+    bands = sorted(rates)
+    assert all(rates[a] >= rates[b] for a, b in zip(bands, bands[1:]))
+    # screen costs alone decide when refining is free
+    free = synthetic_band_entry(refine=(0.0, 0.0))
+    assert rank_bands(p, 4096, 5.5, 1e-2, 16384, entry=free)[0]["band"] == 256
+    # an expensive refine moves the choice to a band that refines less
+    dear = synthetic_band_entry(refine=(1e4, 1e7))
+    pick = rank_bands(p, 4096, 5.5, 1e-2, 16384, entry=dear)[0]["band"]
+    assert rates[pick] <= rates[256]
+    # the matrix-core screen at 512 is cheaper than every FP32 screen
+    assert rank_bands(p, 4096, 5.5, 1e-2, 16384, "fp16", entry=free)[0]["band"] == 512
+    # at 256 pairs every screen costs the same fixed 4 ms: the smallest refine bill wins
+    small = rank_bands(p, 4096, 5.5, 1e-2, 256, entry=synthetic_band_entry(refine=(2.0, 2.0)))
+    assert small[0]["noise_refine_rate"] == min(c["noise_refine_rate"] for c in small)
+
+
+def test_band_choice_comes_from_the_measured_table(fake, monkeypatch):
+    from apogee_amd import autotune
+    monkeypatch.setattr(autotune, "_table", {"entries": [synthetic_band_entry(refine=(0.0, 0.0))]})
+    h = mf.HierarchicalFilter(4096, 16, 1024, snr=5.5, fd=1e-2)
+    h.set_reference(inspiral_power(4096))
+    assert h.config == (256, 8)
+    h.set_coarse_precision("fp16")
+    assert h.config == (512, 8)
+    g = mf.HierarchicalFilter(2048, 2, 2)          # no entry for this length: n / 8
+    g.set_reference(inspiral_power(2048))
+    assert g.config == (256, 8)
+
+
+def test_noise_exceedance_model():
+    from apogee_amd import gatemodel as G
+    n = 4096
+    white = np.ones(n)
+    # white profile: every one of the band lags independent
+    for band, gate in ((512, 4.0), (1024, 5.0)):
+        want = 1 - (1 - np.exp(-gate * gate / 2)) ** band
+        assert abs(G.noise_exceedance(white, n, band, gate) - want) < 1e-12
+    # a narrower profile has fewer independent lags: it exceeds less often
+    assert G.noise_exceedance(inspiral_power(n), n, 512, 4.0) < G.noise_exceedance(white, n, 512, 4.0)
+    assert G.noise_exceedance(white, n, 512, 0.0) == 1.0

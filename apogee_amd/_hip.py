@@ -359,6 +359,16 @@ class Context:
         """:func:`device_view` of ``producer`` ordered on this context's stream."""
         return device_view(producer, self.stream())
 
+    def bind_tensor(self, kind, tensor):
+        """Bind a whole bank to a contiguous complex64 PyTorch tensor of
+        shape (rows, n) in place.  The caller has finished the tensor's
+        producer (a host wait on its stream); the tensor is kept until the
+        bank is replaced."""
+        L = lib()
+        fn = L.mfh_bind_data if kind == "data" else L.mfh_bind_templates
+        check(fn(self.handle, ctypes.c_void_p(int(tensor.data_ptr()))), "bind %s" % kind)
+        self._bound[kind] = tensor
+
     # ---------------------------------------------------------------- runs
     def peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, counts=False):
         """Flat filter over rows [d0, d0+nd) x [t0, t0+nt); host results.

@@ -92,15 +92,20 @@ class PeakFilter:
         return _Peaks.apply(data, templates, self, binsize, float(threshold), start, end)
 
     # ------------------------------------------------------------ kernels
+    # Ordering against PyTorch is by host waits: PyTorch's stream is drained
+    # before a launch reads its tensors, and the context's stream before the
+    # outputs are handed back.  The banks are bound to the input tensors in
+    # place (no copies).
     def _forward(self, data, templates, binsize, threshold, start, end):
-        # read in place: the banks are bound to the tensors' memory, and the
-        # context's stream is ordered after PyTorch's by the DLPack hand-over
-        self._ctx.bind_rows("data", self._ctx.borrow(data.detach().contiguous()))
-        self._ctx.bind_rows("template", self._ctx.borrow(templates.detach().contiguous()))
+        d = data.detach().contiguous()
+        t = templates.detach().contiguous()
         nb = -(-(end - start) // binsize)
         shape = (self.ndata, self.ntemplates, nb)
         index = torch.empty(shape, dtype=torch.int64, device=self._torch_device)
         value = torch.empty(shape, dtype=torch.complex64, device=self._torch_device)
+        torch.cuda.current_stream(self._torch_device).synchronize()
+        self._ctx.bind_tensor("data", d)
+        self._ctx.bind_tensor("template", t)
         self._ctx.peaks_async(0, self.ndata, 0, self.ntemplates, binsize, threshold, start, end,
                               index.data_ptr(), value.data_ptr())
         self._ctx.sync()
@@ -112,14 +117,12 @@ class PeakFilter:
         g = gvalue.to(torch.complex64).contiguous()
         gd = torch.empty_like(d) if need_d else None
         gt = torch.empty_like(t) if need_t else None
-        # order the context's stream after PyTorch's pending work on the inputs
-        keep = [self._ctx.borrow(x) for x in (d, t, g, index)]
+        torch.cuda.current_stream(self._torch_device).synchronize()
         self._ctx.peak_grad_async(d.data_ptr(), self.ndata, t.data_ptr(), self.ntemplates, index.shape[2],
                                   index.data_ptr(), g.data_ptr(),
                                   gd.data_ptr() if gd is not None else 0,
                                   gt.data_ptr() if gt is not None else 0)
         self._ctx.sync()
-        del keep
         return gd, gt
 
     def recompute(self, data, templates, index):
@@ -130,11 +133,10 @@ class PeakFilter:
         t = templates.detach().contiguous()
         idx = index.contiguous()
         out = torch.empty(idx.shape, dtype=torch.complex64, device=self._torch_device)
-        keep = [self._ctx.borrow(x) for x in (d, t, idx)]
+        torch.cuda.current_stream(self._torch_device).synchronize()
         self._ctx.peak_values_async(d.data_ptr(), self.ndata, t.data_ptr(), self.ntemplates, idx.shape[2],
                                     idx.data_ptr(), out.data_ptr())
         self._ctx.sync()
-        del keep
         return out
 
 

@@ -7,9 +7,11 @@ Tunables (each a key of the library's tuning interface):
   reads and fewer, longer workgroups; fewer rows: more workgroups in flight);
 * ``hier`` -- ``refine_grid``: workgroups of the list-driven refine launch,
   which drains however many pairs the coarse gate passed;
-* ``coarse`` -- ``coarse_rpw`` x ``coarse_tpw``: data rows and templates
-  each wave of the matrix-core coarse screen takes (more of either: the
-  per-wave set-up is shared by more pairs; fewer: more waves in flight);
+* ``coarse`` -- ``coarse_lds`` x ``coarse_rpw`` x ``coarse_tpw``: where the
+  matrix-core coarse screen keeps its DFT constants (registers, or read from
+  a per-workgroup LDS copy: fewer VGPRs, more waves per SIMD), and the data
+  rows and templates each wave takes (more of either: the per-wave set-up
+  is shared by more pairs; fewer: more waves in flight);
 * ``queue`` -- ``grid_per_cu``: persistent workgroups per CU of the
   mixed-length refine queue;
 * ``band`` -- not a launch knob but the measured costs the hierarchical
@@ -37,7 +39,7 @@ TABLE_PATH = os.path.join(_HERE, "tuning_gfx950.json")
 CANDIDATES = {
     "flat": {"dpw": [1, 2, 4, 8, 16]},
     "hier": {"refine_grid": [256, 512, 1024, 2048, 4096]},
-    "coarse": {"coarse_rpw": [1, 2], "coarse_tpw": [2, 4, 8, 16]},
+    "coarse": {"coarse_lds": [0, 1], "coarse_rpw": [1, 2], "coarse_tpw": [1, 2, 4, 8, 16]},
     "queue": {"grid_per_cu": [1, 2]},
 }
 
@@ -184,34 +186,18 @@ def _tune_coarse(device, n, nd, nt, band, reps, rng):
     ctx.hier_set_band(band, -1.0)
     ms = {}
     c = CANDIDATES["coarse"]
-    for rpw, tpw in itertools.product(c["coarse_rpw"], c["coarse_tpw"]):
+    for lds, rpw, tpw in itertools.product(c["coarse_lds"], c["coarse_rpw"], c["coarse_tpw"]):
+        ctx.set_tuning("coarse_lds", lds)
         ctx.set_tuning("coarse_rpw", rpw)
         ctx.set_tuning("coarse_tpw", tpw)
         run = lambda: ctx.hier_coarse_max(0, nd, 0, nt, 0, n, 1)   # the fp16 screen
         run()
-        ms[(rpw, tpw)] = _median_ms(run, reps, ctx, "coarse")
+        ms[(lds, rpw, tpw)] = _median_ms(run, reps, ctx, "coarse")
     best = min(ms, key=ms.get)
     ctx.destroy()
     return {"op": "coarse", "n": n, "ndata": nd, "ntemplates": nt, "band": band,
-            "best": {"coarse_rpw": best[0], "coarse_tpw": best[1]},
-            "ms": {"%dx%d" % k: round(v, 5) for k, v in ms.items()}}
-
-
-def uneven_workload(nreq, seed, device="cuda"):
-    """The uneven-refine plan: n in {1024, 4096, 16384, 65536} with weights
-    .4/.3/.2/.1, binsize n/8, candidates ~ geometric(1/3); spectra are seeded
-    Gaussian complex64 generated on the device.  Returns (pool, plan kwargs)."""
-    import numpy as np
-    import torch
-    rng = np.random.default_rng(seed)
-    n = rng.choice([1024, 4096, 16384, 65536], size=nreq, p=[0.4, 0.3, 0.2, 0.1]).astype(np.int64)
-    cand = rng.geometric(1.0 / 3.0, size=nreq).astype(np.int64)
-    sizes = n * (1 + cand)
-    doff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
-    g = torch.Generator(device=device)
-    g.manual_seed(seed)
-    pool = torch.randn(int(sizes.sum()), dtype=torch.complex64, device=device, generator=g)
-    return pool, dict(n=n, data_offset=doff, template_offset=doff + n, candidates=cand, binsize=n // 8)
+            "best": {"coarse_lds": best[0], "coarse_rpw": best[1], "coarse_tpw": best[2]},
+            "ms": {"%d:%dx%d" % k: round(v, 5) for k, v in ms.items()}}
 
 
 def band_entry(n):
@@ -320,30 +306,44 @@ def main(argv=None):
     ap.add_argument("--out", default=TABLE_PATH)
     ap.add_argument("--reps", type=int, default=7)
     ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--only", default="",
+                    help="comma-separated operations to re-measure (flat, hier, coarse, band, queue); "
+                         "the other entries of the existing table are kept")
     args = ap.parse_args(argv)
+    only = {o for o in args.only.split(",") if o}
     name, arch = _hip.device_info(args.device)
     rng = np.random.default_rng(7)
+    keep = []
+    if only:
+        try:
+            with open(TABLE_PATH) as fh:
+                keep = [e for e in json.load(fh).get("entries", []) if e.get("op") not in only]
+        except (OSError, ValueError):
+            keep = []
+    want = (lambda op: not only or op in only)
     entries = []
-    for n, nd, nt, bs in FLAT_SHAPES:
-        e = _tune_flat(args.device, n, nd, nt, bs, args.reps, rng)
+
+    def add(e):
         print(json.dumps(e), flush=True)
         entries.append(e)
-    for n, nd, nt, bs, band in HIER_SHAPES:
-        e = _tune_hier(args.device, n, nd, nt, bs, band, args.reps, rng)
-        print(json.dumps(e), flush=True)
-        entries.append(e)
-    for n, nd, nt, band in COARSE_SHAPES:
-        e = _tune_coarse(args.device, n, nd, nt, band, args.reps, rng)
-        print(json.dumps(e), flush=True)
-        entries.append(e)
-    for n, nd, nt in BAND_SHAPES:
-        e = _measure_band(args.device, n, nd, nt, args.reps, rng)
-        print(json.dumps(e), flush=True)
-        entries.append(e)
-    for wname, nreq in QUEUE_WORKLOADS:
-        e = _tune_queue(args.device, wname, nreq, args.reps)
-        print(json.dumps(e), flush=True)
-        entries.append(e)
+
+    if want("flat"):
+        for n, nd, nt, bs in FLAT_SHAPES:
+            add(_tune_flat(args.device, n, nd, nt, bs, args.reps, rng))
+    if want("hier"):
+        for n, nd, nt, bs, band in HIER_SHAPES:
+            add(_tune_hier(args.device, n, nd, nt, bs, band, args.reps, rng))
+    if want("coarse"):
+        for n, nd, nt, band in COARSE_SHAPES:
+            add(_tune_coarse(args.device, n, nd, nt, band, args.reps, rng))
+    if want("band"):
+        for n, nd, nt in BAND_SHAPES:
+            add(_measure_band(args.device, n, nd, nt, args.reps, rng))
+    if want("queue"):
+        for wname, nreq in QUEUE_WORKLOADS:
+            add(_tune_queue(args.device, wname, nreq, args.reps))
+    order = ["flat", "hier", "coarse", "band", "queue"]
+    entries = sorted(keep + entries, key=lambda e: order.index(e["op"]) if e["op"] in order else 99)
     table = {"device": name, "arch": arch, "method": "median of %d event-timed launches" % args.reps,
              "entries": entries}
     with open(args.out, "w") as fh:

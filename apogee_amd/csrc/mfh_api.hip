@@ -407,6 +407,11 @@ extern "C" int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value) {
     c->coarse_tpw = (int)value;
     return 0;
   }
+  if (!strcmp(key, "coarse_lds")) {
+    if (value != 0 && value != 1) return fail("coarse_lds must be 0 or 1");
+    c->coarse_lds = (int)value;
+    return 0;
+  }
   if (!strcmp(key, "refine_grid")) {
     if (value < 1) return fail("refine_grid must be >= 1");
     c->refine_grid_cap = (int)value;

@@ -188,6 +188,75 @@ struct CoarseDft {
     }
   }
 
+  // The constants straight from a workgroup's LDS copy of the table, stored
+  // word4-major ([w][lane], so a wave's 16-byte reads are conflict-free),
+  // instead of held in registers: steps' constants are read where used.
+  __device__ __forceinline__ static void lds_words(uint4 *w, const uint4 *tabT, int lane, int first, int count) {
+#pragma unroll
+    for (int i = 0; i < 12; i++)
+      if (i >= first && i < first + count) w[i] = tabT[i * 64 + lane];
+  }
+  template <bool FULL>
+  __device__ __forceinline__ uint32_t lane_max_lds(const uint4 *tabT, int lane, const cf *x, const cf *ct, int cs,
+                                                   int ce, int q, int r) {
+    // an opaque lane index per call keeps the reads inside the template loop
+    int lo = lane;
+    asm volatile("" : "+v"(lo));
+    uint4 w[12];
+    lds_words(w, tabT, lo, 10, 2);
+    __builtin_memcpy(&gr, &w[10], 8);
+    __builtin_memcpy(&gi, reinterpret_cast<const char *>(&w[10]) + 8, 8);
+    __builtin_memcpy(&gn, &w[11], 8);
+    typename F::v8 xr, xi;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const cf p = cmul_conj(x[i], ct[i]);
+      xr[i] = F::cvt8(p.x);
+      xi[i] = F::cvt8(p.y);
+    }
+    uint32_t best = 0u;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      w[h] = tabT[h * 64 + lo];
+      w[2 + h] = tabT[(2 + h) * 64 + lo];
+      w[4 + h] = tabT[(4 + h) * 64 + lo];
+      w[6 + 2 * h] = tabT[(6 + 2 * h) * 64 + lo];
+      w[7 + 2 * h] = tabT[(7 + 2 * h) * 64 + lo];
+      __builtin_memcpy(&fr[h], &w[h], 16);
+      __builtin_memcpy(&fi[h], &w[2 + h], 16);
+      __builtin_memcpy(&fn[h], &w[4 + h], 16);
+      __builtin_memcpy(&tw[h][0], &w[6 + 2 * h], 32);
+      float4v yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
+      yr = mfma16x16x32(xr, fr[h], yr);
+      yr = mfma16x16x32(xi, fn[h], yr);
+      yi = mfma16x16x32(xr, fi[h], yi);
+      yi = mfma16x16x32(xi, fr[h], yi);
+      typename F::v4 br, bi;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const cf y = cmul(cf{yr[i], yi[i]}, tw[h][i]);
+        br[i] = F::cvt4(y.x);
+        bi[i] = F::cvt4(y.y);
+      }
+      float4v zr = {0.f, 0.f, 0.f, 0.f}, zi = {0.f, 0.f, 0.f, 0.f};
+      zr = mfma16x16x16(gr, br, zr);
+      zr = mfma16x16x16(gn, bi, zr);
+      zi = mfma16x16x16(gr, bi, zi);
+      zi = mfma16x16x16(gi, br, zi);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const uint32_t m2 = __float_as_uint(__builtin_fmaf(zr[i], zr[i], zi[i] * zi[i]));
+        if (FULL) {
+          best = max(best, m2);
+        } else {
+          const int j = r + 16 * h + 32 * (4 * q + i);
+          if (j >= cs && j < ce) best = max(best, m2);
+        }
+      }
+    }
+    return best;
+  }
+
   // This lane's maximum of |z[j]|^2 over its 8 lags j = r + 16 h + 32 (4q + i)
   // that lie in [cs, ce) (all of them when FULL), for the scaled spectra x
   // (data) and ct (template) at the A-fragment positions.  |z|^2 >= 0, so
@@ -254,11 +323,31 @@ __device__ __forceinline__ void load_frag8(cf *ct, const cf *ctm, int t, int lan
 // |Y| < 32 and |z| < 512: no fp16 overflow.  Writes the coarse maximum
 // |z_c|^2 over lags [cs, ce) per pair, in units of the unscaled inputs (an
 // exact power-of-two rescale).
-template <bool BF16, int R, bool FULL>
+//
+// LDSC: the DFT constants are copied once per workgroup into LDS and read
+// where used instead of held in registers (46 fewer VGPRs per lane, so more
+// waves fit per SIMD).
+template <bool BF16, int R, bool FULL, bool LDSC>
 __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long long data_stride, const cf *ctm,
                                                           const int *sexp, int nd, int nt, int tpw, int cs, int ce,
                                                           const uint32_t *consts, float *cmax2, CoarseGate g) {
   const int lane = threadIdx.x & 63;
+  __shared__ uint4 ctab[LDSC ? 12 * 64 : 1];
+  // LDSC: the workgroup's templates' fragments, [template][quad][lane] in
+  // 16-byte units (conflict-free reads), staged once for its four waves
+  extern __shared__ uint4 tfrag[];
+  if constexpr (LDSC) {
+    // global table: lane-major, 12 words4 per lane; LDS: word4-major
+    const uint4 *src = reinterpret_cast<const uint4 *>(consts);
+    for (int i = threadIdx.x; i < 12 * 64; i += 256) ctab[(i % 12) * 64 + i / 12] = src[i];
+    const int tw0 = blockIdx.x * tpw, tcnt = min(nt, tw0 + tpw) - tw0;
+    const uint4 *tsrc = reinterpret_cast<const uint4 *>(ctm) + (long long)tw0 * 256;
+    for (int i = threadIdx.x; i < tcnt * 256; i += 256) {
+      const int j = i >> 8, l = (i >> 2) & 63, q4 = i & 3;
+      tfrag[(j * 4 + q4) * 64 + l] = tsrc[i];
+    }
+    __syncthreads();
+  }
   const int d0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * R;
   if (g.zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *g.zero = 0;
   // a wave past the last data row walks no templates but stays for the
@@ -285,7 +374,7 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
     for (int i = 0; i < 8; i++) x[k][i] = cf{ldexpf(x[k][i].x, -sd[k]), ldexpf(x[k][i].y, -sd[k])};
   }
   CoarseDft<BF16> dft;
-  dft.load_table(consts, lane);
+  if constexpr (!LDSC) dft.load_table(consts, lane);
 
   // template t's pairs: the transforms of all R rows first (independent
   // chains the scheduler interleaves), then the reductions; gate decisions
@@ -297,7 +386,12 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
   auto pairs = [&](const cf *ct, int t) {
     uint32_t lm[R];
 #pragma unroll
-    for (int k = 0; k < R; k++) lm[k] = dft.template lane_max<FULL>(x[k], ct, cs, ce, q, r);
+    for (int k = 0; k < R; k++) {
+      if constexpr (LDSC)
+        lm[k] = dft.template lane_max_lds<FULL>(ctab, lane, x[k], ct, cs, ce, q, r);
+      else
+        lm[k] = dft.template lane_max<FULL>(x[k], ct, cs, ce, q, r);
+    }
     const int st = sexp[t];
 #pragma unroll
     for (int k = 0; k < R; k++) {
@@ -308,16 +402,29 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
     }
   };
 
-  // two fragment buffers in turn: the next template's loads are in flight
-  // while the current one is transformed
-  cf ca[8], cb[8];
-  if (t0 < t1) load_frag8(ca, ctm, t0, lane);
-  for (int t = t0; t < t1; t += 2) {
-    if (t + 1 < t1) load_frag8(cb, ctm, t + 1, lane);
-    pairs(ca, t);
-    if (t + 1 >= t1) break;
-    if (t + 2 < t1) load_frag8(ca, ctm, t + 2, lane);
-    pairs(cb, t + 1);
+  if constexpr (LDSC) {
+    for (int t = t0; t < t1; t++) {
+      cf ct[8];
+#pragma unroll
+      for (int q4 = 0; q4 < 4; q4++) {
+        const uint4 v = tfrag[((t - t0) * 4 + q4) * 64 + lane];
+        ct[2 * q4] = cf{__uint_as_float(v.x), __uint_as_float(v.y)};
+        ct[2 * q4 + 1] = cf{__uint_as_float(v.z), __uint_as_float(v.w)};
+      }
+      pairs(ct, t);
+    }
+  } else {
+    // two fragment buffers in turn: the next template's loads are in flight
+    // while the current one is transformed
+    cf ca[8], cb[8];
+    if (t0 < t1) load_frag8(ca, ctm, t0, lane);
+    for (int t = t0; t < t1; t += 2) {
+      if (t + 1 < t1) load_frag8(cb, ctm, t + 1, lane);
+      pairs(ca, t);
+      if (t + 1 >= t1) break;
+      if (t + 2 < t1) load_frag8(ca, ctm, t + 2, lane);
+      pairs(cb, t + 1);
+    }
   }
   if (!g.list) return;  // uniform over the grid
 
@@ -425,17 +532,27 @@ int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, int t0, in
   const cf *ctm = c->coarse_ctm + (size_t)t0 * kBand;
   const int *sexp = c->coarse_sexp + t0;
   const bool full = cs == 0 && ce == kBand;  // whole coarse window: no per-lag test
-  decltype(&coarse_mfma_kernel<true, 1, true>) kern;
-  if (bf16)
-    kern = rows == 1 ? (full ? coarse_mfma_kernel<true, 1, true> : coarse_mfma_kernel<true, 1, false>)
-                     : (full ? coarse_mfma_kernel<true, 2, true> : coarse_mfma_kernel<true, 2, false>);
-  else
-    kern = rows == 1 ? (full ? coarse_mfma_kernel<false, 1, true> : coarse_mfma_kernel<false, 1, false>)
-                     : (full ? coarse_mfma_kernel<false, 2, true> : coarse_mfma_kernel<false, 2, false>);
+  using Kern = decltype(&coarse_mfma_kernel<true, 1, true, false>);
+  // [bf16][rows - 1][full][lds]
+  static const Kern table[2][2][2][2] = {
+      {{{coarse_mfma_kernel<false, 1, false, false>, coarse_mfma_kernel<false, 1, false, true>},
+        {coarse_mfma_kernel<false, 1, true, false>, coarse_mfma_kernel<false, 1, true, true>}},
+       {{coarse_mfma_kernel<false, 2, false, false>, coarse_mfma_kernel<false, 2, false, true>},
+        {coarse_mfma_kernel<false, 2, true, false>, coarse_mfma_kernel<false, 2, true, true>}}},
+      {{{coarse_mfma_kernel<true, 1, false, false>, coarse_mfma_kernel<true, 1, false, true>},
+        {coarse_mfma_kernel<true, 1, true, false>, coarse_mfma_kernel<true, 1, true, true>}},
+       {{coarse_mfma_kernel<true, 2, false, false>, coarse_mfma_kernel<true, 2, false, true>},
+        {coarse_mfma_kernel<true, 2, true, false>, coarse_mfma_kernel<true, 2, true, true>}}}};
+  // LDS staging holds at most 16 templates (64 KiB); wider tiles keep the
+  // constants and fragments in registers
+  const bool lds = c->coarse_lds && tpw <= 16;
+  const Kern kern = table[bf16 ? 1 : 0][rows - 1][full ? 1 : 0][lds ? 1 : 0];
+  const size_t smem = lds ? (size_t)tpw * 4096 : 0;
+  if (lds && !lds_limit((const void *)kern, c->dev, smem)) return -1;
   Ctx::ProfScope ps(c, "coarse");
   const CoarseGate g = gate ? *gate : CoarseGate{};
-  hipLaunchKernelGGL(kern, grid, dim3(256), 0, c->stream, data, data_stride, ctm, sexp, nd, nt, tpw, cs, ce, consts,
-                     cmax2, g);
+  hipLaunchKernelGGL(kern, grid, dim3(256), smem, c->stream, data, data_stride, ctm, sexp, nd, nt, tpw, cs, ce,
+                     consts, cmax2, g);
   return hip_ok(hipGetLastError(), "coarse_mfma_kernel") ? 0 : -1;
 }
 

@@ -383,7 +383,7 @@ int hier_run_cached(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, floa
   std::vector<long long> key = {d0, nd, t0, nt, (long long)binsize, fbits(threshold), (long long)ws, (long long)we,
                                 fbits(coarse_thr), (long long)(intptr_t)idx, (long long)(intptr_t)val,
                                 c->coarse_band, dbits(c->coarse_ref_f), c->coarse_prec, fbits(c->coarse_delta),
-                                c->refine_grid_cap, (long long)(intptr_t)c->coarse_tmpl, c->coarse_tpw, c->coarse_rpw,
+                                c->refine_grid_cap, (long long)(intptr_t)c->coarse_tmpl, c->coarse_tpw, c->coarse_rpw, c->coarse_lds,
                                 (long long)(intptr_t)c->data, (long long)(intptr_t)c->tmpl};
   const int par = c->hier_parity;
   if (key == c->graph_key && c->hier_graph[par]) {

@@ -50,6 +50,7 @@ struct Ctx {
   float coarse_delta = 0.f;
   int coarse_tpw = 4;  // templates per wave of the MFMA coarse screen
   int coarse_rpw = 2;  // data rows per wave of the MFMA coarse screen (1 or 2)
+  int coarse_lds = 0;  // 1: the screen reads its DFT constants from LDS, not registers
   // the coarse templates in MFMA fragment order, power-of-two scaled, and
   // their scale exponents (built with coarse_tmpl at band 512)
   cf *coarse_ctm = nullptr;

@@ -272,6 +272,23 @@ def _measure_band(device, n, nd, nt, reps, rng):
             "pairs": [snd * snt for snd, snt in shapes], "flat_ms": flat, "bands": bands}
 
 
+def uneven_workload(nreq, seed, device="cuda"):
+    """The uneven-refine plan: n in {1024, 4096, 16384, 65536} with weights
+    .4/.3/.2/.1, binsize n/8, candidates ~ geometric(1/3); spectra are seeded
+    Gaussian complex64 generated on the device.  Returns (pool, plan kwargs)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    n = rng.choice([1024, 4096, 16384, 65536], size=nreq, p=[0.4, 0.3, 0.2, 0.1]).astype(np.int64)
+    cand = rng.geometric(1.0 / 3.0, size=nreq).astype(np.int64)
+    sizes = n * (1 + cand)
+    doff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    pool = torch.randn(int(sizes.sum()), dtype=torch.complex64, device=device, generator=g)
+    return pool, dict(n=n, data_offset=doff, template_offset=doff + n, candidates=cand, binsize=n // 8)
+
+
 def _tune_queue(device, name, nreq, reps):
     from .refine import RefineQueue
     import torch

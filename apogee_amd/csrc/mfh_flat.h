@@ -232,13 +232,19 @@ __device__ __forceinline__ bool one_bin_wave_key(const cf *v, int t, float seed,
   uint32_t m = mb[0];
 #pragma unroll
   for (int u = 1; u < 16; u++) m = max(m, mb[u]);
+  const uint32_t mw = wave_umax(m);
+  if (mw > 0x7f800000u) return false;  // wave-uniform
+  // a wave whose maximum does not beat the seed raises no key, and skips
+  // finding its lag (wave-uniform): under a threshold most waves stop here
+  if (mw <= __float_as_uint(seed)) {
+    wkey = 0ull;
+    return true;
+  }
   int bu = 15;
 #pragma unroll
   for (int u = 14; u >= 0; u--) bu = mb[u] == m ? u : bu;
-  const uint32_t mw = wave_umax(m);
-  if (mw > 0x7f800000u) return false;  // wave-uniform
   const uint32_t kw = wave_umin(m == mw ? (uint32_t)(t + bu * TPT) : 0xffffffffu);
-  wkey = mw > __float_as_uint(seed) ? ((uint64_t)mw << 32) | (uint64_t)(~kw) : 0ull;
+  wkey = ((uint64_t)mw << 32) | (uint64_t)(~kw);
   return true;
 }
 

@@ -108,3 +108,17 @@ def test_add_after_run_replans(fake):
     assert [o.shape for o in out] == [(2, 1), (1, 4)]
     q.clear()
     assert len(q) == 0
+
+
+def test_uneven_workload_recipe():
+    """The benchmark's uneven-refine plan (autotune.uneven_workload) on the
+    host: lengths from the four classes, binsize n/8, offsets that tile the
+    pool exactly."""
+    from apogee_amd.autotune import uneven_workload
+    pool, plan = uneven_workload(200, 5, device="cpu")
+    n, cand = plan["n"], plan["candidates"]
+    assert set(np.unique(n)) <= {1024, 4096, 16384, 65536} and (cand >= 1).all()
+    assert (plan["binsize"] == n // 8).all()
+    assert (plan["template_offset"] == plan["data_offset"] + n).all()
+    assert int(pool.numel()) == int((n * (1 + cand)).sum())
+    assert (plan["data_offset"][1:] == plan["template_offset"][:-1] + cand[:-1] * n[:-1]).all()

@@ -66,42 +66,60 @@ __device__ __forceinline__ bool big_pair(const BigArgs &a, long long p, int &d, 
   return true;
 }
 
+// w_n^m for 0 <= m < n from the two root tables (global, a few KiB, cached).
+__device__ __forceinline__ cf big_root(const BigArgs &a, int m) { return cmul(a.rootH[m >> 8], a.rootL[m & 255]); }
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
 // Columns: 16 columns f1 of one pair per workgroup; lane = 16 * t + s, so the
 // 16 lanes of a row group touch 16 consecutive columns (128-byte accesses).
+// The pair's rows and its Y block are wave-uniform, so loads and stores go
+// through buffer descriptors: one vector offset per thread and the per-u
+// stride as a scalar offset, with no vector address arithmetic per point.
 template <int N1>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) big_cols_kernel(BigArgs a) {
   constexpr int N = N1 * 256;
-  using S256 = FlatShape<256>;
   extern __shared__ __align__(16) unsigned char smem[];
   const int s = threadIdx.x & 15;
   const int t = threadIdx.x >> 4;
   cf *xch = reinterpret_cast<cf *>(smem) + s * kColStride;
-  // the two root tables, copied to LDS: the twiddle gathers below are
-  // scattered, and LDS serves them far faster than the vector cache
-  cf *rH = reinterpret_cast<cf *>(smem) + 16 * kColStride;
-  cf *rL = rH + N1;
+  // The closing twiddle of output k2 = t + 16 u of column f1 splits as
+  // w_n^(f1 k2) = w_n^(f1 t) * w_n^(16 f1 u): the first factor is one per
+  // thread, the second one per (column, u), 256 per workgroup, kept in LDS
+  // as bt[17 s + u] (rows of 17: the 16 columns a wave reads in one step
+  // fall in distinct banks, its 4 row groups read the same words).
+  cf *bt = reinterpret_cast<cf *>(smem) + 16 * kColStride;
   const long long pl = blockIdx.y;  // pair within the chunk
   int d = 0, tt = 0;
   const bool valid = big_pair(a, a.p0 + pl, d, tt);  // uniform across the workgroup
   if (!valid) return;                                 // no barrier has run yet
-  for (int i = threadIdx.x; i < N1 + 256; i += 256) rH[i] = i < N1 ? a.rootH[i] : a.rootL[i - N1];
   const int f1 = blockIdx.x * 16 + s;
-  const cf *D = a.data + (long long)d * a.data_stride;
-  const cf *T = a.tmpl + (long long)tt * a.tmpl_stride;
+  bt[17 * s + t] = big_root(a, 16 * f1 * t);  // u = t; 16 f1 t < n
+  const cf wt = big_root(a, f1 * t);
+  const cf *D = uniform_ptr(a.data + (long long)d * a.data_stride);
+  const cf *T = uniform_ptr(a.tmpl + (long long)tt * a.tmpl_stride);
+  const __amdgpu_buffer_rsrc_t rd =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(D), (short)0, N * (int)sizeof(cf), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rt =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<cf *>(T), (short)0, N * (int)sizeof(cf), 0x00020000);
+  const int voff = (f1 + N1 * t) * (int)sizeof(cf);  // element f1 + N1 (t + 16 u) at voff + u * 16 N1 * 8
   cf v[16];
 #pragma unroll
   for (int u = 0; u < 16; u++) {
-    const int f = f1 + N1 * (t + 16 * u);
-    v[u] = cmul_conj(D[f], T[f]);
+    const int so = u * 16 * N1 * (int)sizeof(cf);
+    const cf x = __builtin_bit_cast(cf, __builtin_amdgcn_raw_buffer_load_b64(rd, voff, so, 0));
+    const cf y = __builtin_bit_cast(cf, __builtin_amdgcn_raw_buffer_load_b64(rt, voff, so, 0));
+    v[u] = cmul_conj(x, y);
   }
-  inverse_fft<256>(v, t, xch, a.tw256);  // its barriers order the table copy before the reads
-  cf *Y = a.scratch + pl * N;
+  inverse_fft<256>(v, t, xch, a.tw256);  // its barriers order the bt stores before the reads
+  cf *Y = const_cast<cf *>(uniform_ptr(a.scratch + pl * N));
+  const __amdgpu_buffer_rsrc_t ry = __builtin_amdgcn_make_buffer_rsrc(Y, (short)0, N * (int)sizeof(cf), 0x00020000);
+  const int yoff = (t * N1 + f1) * (int)sizeof(cf);  // Y[k2][f1], k2 = t + 16 u
 #pragma unroll
   for (int u = 0; u < 16; u++) {
-    const int k2 = t + 16 * u;
-    const int m = f1 * k2;                         // < N
-    const cf w = cmul(rH[m >> 8], rL[m & 255]);
-    Y[(long long)k2 * N1 + f1] = cmul(v[u], w);
+    const cf w = cmul(wt, bt[17 * s + u]);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, cmul(v[u], w)), ry, yoff,
+                                          u * 16 * N1 * (int)sizeof(cf), 0);
   }
 }
 
@@ -130,21 +148,45 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
   const cf *Y = a.scratch + pl * (long long)(N1 * 256);
   constexpr int G = 256 / SLOTS;  // row groups of the pair
   const int gbeg = (int)blockIdx.y * G / a.nsplit, gend = ((int)blockIdx.y + 1) * G / a.nsplit;
-  const bool full8 = a.ws == 0 && a.we == (uint32_t)(N1 * 256) && a.bshift == ilog2(N1 * 256 / 8);
+  const bool whole = a.ws == 0 && a.we == (uint32_t)(N1 * 256);
+  const bool full8 = whole && a.bshift == ilog2(N1 * 256 / 8);
   FlatArgs fa;  // for bin_of / make_key
   fa.ws = a.ws;
   fa.we = a.we;
   fa.binsize = a.binsize;
   fa.bshift = a.bshift;
   fa.inv_bs = a.inv_bs;
+  // the first twiddled pass's twiddles depend on t only: derived once for
+  // all the part's row groups where the registers allow (N1 <= 512)
+  constexpr bool HOIST = N1 <= 512;
+  cf wf[HOIST ? 15 : 1];
+  if constexpr (HOIST) first_twiddles<N1>(wf, t, a.twN1);
   for (int g = gbeg; g < gend; g++) {
     const int k2 = g * SLOTS + slot;
     cf v[16];
 #pragma unroll
     for (int u = 0; u < 16; u++) v[u] = Y[(long long)k2 * N1 + t + TPT * u];
-    inverse_fft<N1>(v, t, xch, a.twN1);
-    // thread t holds z[k2 + 256 (t + TPT u)]; lags rise with u
-    if (full8) {
+    inverse_fft<N1, HOIST>(v, t, xch, a.twN1, wf);
+    // thread t holds z[k2 + 256 (t + TPT u)]; lags rise with u.  A wave none
+    // of whose window lags has |z|^2 above the seed raises no key and records
+    // no value, so it skips both steps below (wave-uniform; nonnegative
+    // floats order like their bits, and a NaN's bits, above all of them,
+    // send the wave through the steps, which never select it)
+    uint32_t mloc = 0;
+    if (whole) {
+#pragma unroll
+      for (int u = 0; u < 16; u++) mloc = max(mloc, __float_as_uint(__builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y)));
+    } else {
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const uint32_t k = (uint32_t)(k2 + 256 * (t + TPT * u));
+        const uint32_t m = __float_as_uint(__builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y));
+        mloc = (k >= a.ws && k < a.we) ? max(mloc, m) : mloc;
+      }
+    }
+    const bool hot = wave_umax(mloc) > __float_as_uint(a.seed);
+    if (!hot) {
+    } else if (full8) {
       // n/8 bins over the whole transform: every wave's 64 lanes put u and
       // u + 1 = 2 rho, 2 rho + 1 in bin rho (their lags span < 256 TPT)
       const int lane = threadIdx.x & 63;
@@ -171,14 +213,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
     }
     __syncthreads();
     // the holder of each bin's key records its value
+    if (hot) {
 #pragma unroll
-    for (int u = 0; u < 16; u++) {
-      const uint32_t k = (uint32_t)(k2 + 256 * (t + TPT * u));
-      if (k >= a.ws && k < a.we) {
-        const uint64_t key = make_key(v[u], k, a.seed);
-        if (key) {
-          const uint32_t b = bin_of(k, fa);
-          if (bkey[b] == key) bval[b] = v[u];
+      for (int u = 0; u < 16; u++) {
+        const uint32_t k = (uint32_t)(k2 + 256 * (t + TPT * u));
+        if (k >= a.ws && k < a.we) {
+          const uint64_t key = make_key(v[u], k, a.seed);
+          if (key) {
+            const uint32_t b = bin_of(k, fa);
+            if (bkey[b] == key) bval[b] = v[u];
+          }
         }
       }
     }
@@ -210,6 +254,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
 
 // One thread per (pair, bin) of a chunk: the largest key over the parts
 // (keys are distinct per lag, so exactly one part holds it) and its value.
+// (Merging in the rows kernel instead -- the last part of a pair to finish,
+// found by a ticket counter -- needs device-scope fences around the ticket,
+// and measured 3.5x slower for the whole rows launch on gfx950.)
 __global__ void __launch_bounds__(256) big_rows_finish_kernel(BigArgs a, long long np) {
   const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
   const int nb = (int)a.nb;
@@ -277,7 +324,7 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
   cf *scr = (cf *)c->scratch("big_scratch", (size_t)chunk * N * sizeof(cf));
   if (!scr) return -1;
   a.scratch = scr;
-  const size_t lds_cols = (16 * kColStride + N1 + 256) * sizeof(cf);
+  const size_t lds_cols = (16 * kColStride + 16 * 17) * sizeof(cf);
   const float thr = r.threshold;
   a.seed = thr > 0.f ? (float)((double)thr * (double)thr) : 0.f;
   a.binsize = (uint32_t)r.binsize;

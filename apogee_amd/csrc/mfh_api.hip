@@ -63,6 +63,50 @@ Ctx::~Ctx() {
   }
   for (auto e : event_pool) hipEventDestroy(e);
   if (stream) hipStreamDestroy(stream);
+  if (stage) hipHostFree(stage);
+}
+
+void *Ctx::host_stage(size_t bytes) {
+  if (stage && stage_bytes >= bytes) return stage;
+  if (stage) {
+    hipStreamSynchronize(stream);
+    hipHostFree(stage);
+    stage = nullptr;
+    stage_bytes = 0;
+  }
+  if (!hip_ok(hipHostMalloc(&stage, bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc")) {
+    stage = nullptr;
+    return nullptr;
+  }
+  stage_bytes = bytes;
+  return stage;
+}
+
+bool peak_buffer(Ctx *c, size_t cells, PeakBuf &b) {
+  char *p = (char *)c->scratch("out_peaks", cells * (sizeof(int64_t) + sizeof(cf)));
+  if (!p) return false;
+  b.idx = (int64_t *)p;
+  b.val = (cf *)(p + cells * sizeof(int64_t));
+  return true;
+}
+
+long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts) {
+  const size_t bi = cells * sizeof(int64_t), bytes = bi + cells * sizeof(cf);
+  char *st = (char *)c->host_stage(bytes);
+  if (!st) return -1;
+  if (!hip_ok(hipMemcpyAsync(st, b.idx, bytes, hipMemcpyDeviceToHost, c->stream), "copy results") ||
+      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+    return -1;
+  memcpy(idx, st, bi);
+  memcpy(val_c64, st + bi, bytes - bi);
+  long total = 0;
+  for (size_t p = 0; p < cells / nb; p++) {
+    int cnt = 0;
+    for (size_t k = 0; k < nb; k++) cnt += idx[p * nb + k] >= 0;
+    if (counts) counts[p] = cnt;
+    total += cnt;
+  }
+  return total;
 }
 
 const cf *Ctx::twiddles(size_t len) {
@@ -361,22 +405,11 @@ extern "C" long mfh_flat_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, siz
   if (nb == 0) return fail("empty window (%zu, %zu)", ws, we);
   const size_t cells = (size_t)nd * (size_t)nt * nb;
   hipSetDevice(c->dev);
-  int64_t *didx = (int64_t *)c->scratch("out_idx", cells * sizeof(int64_t));
-  cf *dval = (cf *)c->scratch("out_val", cells * sizeof(cf));
-  if (!didx || !dval) return -1;
-  if (mfh_flat_peaks_async(ctx, d0, nd, t0, nt, binsize, threshold, ws, we, didx, dval) < 0) return -1;
-  if (!hip_ok(hipMemcpyAsync(idx, didx, cells * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream), "copy idx") ||
-      !hip_ok(hipMemcpyAsync(val_c64, dval, cells * sizeof(cf), hipMemcpyDeviceToHost, c->stream), "copy val") ||
-      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
-    return -1;
+  mfh::PeakBuf pb;
+  if (!mfh::peak_buffer(c, cells, pb)) return -1;
+  if (mfh_flat_peaks_async(ctx, d0, nd, t0, nt, binsize, threshold, ws, we, pb.idx, pb.val) < 0) return -1;
+  const long total = mfh::read_peaks(c, pb, cells, nb, idx, val_c64, counts);
   c->prof_collect();
-  long total = 0;
-  for (size_t p = 0; p < (size_t)nd * nt; p++) {
-    int cnt = 0;
-    for (size_t b = 0; b < nb; b++) cnt += idx[p * nb + b] >= 0;
-    if (counts) counts[p] = cnt;
-    total += cnt;
-  }
   return total;
 }
 

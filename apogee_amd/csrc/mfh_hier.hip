@@ -450,26 +450,15 @@ extern "C" long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, siz
   if (nd < 1 || nt < 1) return fail("data/templates sub-range out of bounds");
   const size_t cells = (size_t)nd * (size_t)nt * nb;
   hipSetDevice(c->dev);
-  int64_t *didx = (int64_t *)c->scratch("out_idx", cells * sizeof(int64_t));
-  mfh::cf *dval = (mfh::cf *)c->scratch("out_val", cells * sizeof(mfh::cf));
-  if (!didx || !dval) return -1;
+  mfh::PeakBuf pb;
+  if (!mfh::peak_buffer(c, cells, pb)) return -1;
   int refined = 0;
-  if (mfh::hier_run_cached(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, didx, dval, &refined) < 0)
+  if (mfh::hier_run_cached(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, pb.idx, pb.val, &refined) < 0)
     return -1;
-  if (!hip_ok(hipMemcpyAsync(idx, didx, cells * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream), "copy idx") ||
-      !hip_ok(hipMemcpyAsync(val_c64, dval, cells * sizeof(mfh::cf), hipMemcpyDeviceToHost, c->stream),
-              "copy val") ||
-      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
-    return -1;
+  const long total = mfh::read_peaks(c, pb, cells, nb, idx, val_c64, counts);
+  if (total < 0) return -1;
   c->prof_collect();
   if (n_refined) *n_refined = refined;
-  long total = 0;
-  for (size_t p = 0; p < (size_t)nd * nt; p++) {
-    int cnt = 0;
-    for (size_t b = 0; b < nb; b++) cnt += idx[p * nb + b] >= 0;
-    if (counts) counts[p] = cnt;
-    total += cnt;
-  }
   return total;
 }
 

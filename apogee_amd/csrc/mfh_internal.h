@@ -65,6 +65,12 @@ struct Ctx {
   std::map<size_t, cf *> tw_by_n;
   const cf *twiddles(size_t n);
 
+  // Pinned host staging for results read back into caller host arrays,
+  // grown on demand (hipHostMalloc; contents not preserved on growth).
+  void *stage = nullptr;
+  size_t stage_bytes = 0;
+  void *host_stage(size_t bytes);
+
   struct Buf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -119,6 +125,19 @@ struct Ctx {
 
   ~Ctx();
 };
+
+// A call's device results, laid out [cells indices | cells values] in the
+// context's "out_peaks" buffer so that one copy brings both back.
+struct PeakBuf {
+  int64_t *idx = nullptr;
+  cf *val = nullptr;
+};
+bool peak_buffer(Ctx *c, size_t cells, PeakBuf &b);
+// Copy `cells` results into the caller's host arrays -- one device-to-host
+// copy into the pinned staging buffer, then host copies -- after waiting for
+// the context's stream; counts[p] (if set) = bins set of pair p, nb bins per
+// pair.  Returns the total of set bins, or -1 (message set).
+long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts);
 
 // One flat launch: pairs (data rows x template rows) from the given bases.
 struct FlatRun {

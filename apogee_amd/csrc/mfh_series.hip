@@ -200,9 +200,10 @@ long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int s
   cf *spec = (cf *)c->scratch("series_spec", (size_t)nblocks * n * sizeof(cf));
   long long *dstart = (long long *)c->scratch("series_starts", (size_t)nblocks * sizeof(long long));
   const size_t cells = (size_t)nblocks * nt * nb;
-  int64_t *didx = (int64_t *)c->scratch("out_idx", cells * sizeof(int64_t));
-  cf *dval = (cf *)c->scratch("out_val", cells * sizeof(cf));
-  if (!spec || !dstart || !didx || !dval) return -1;
+  mfh::PeakBuf pb;
+  if (!spec || !dstart || !mfh::peak_buffer(c, cells, pb)) return -1;
+  int64_t *didx = pb.idx;
+  cf *dval = pb.val;
   const cf *dser;
   if (series_on_device) {
     dser = (const cf *)series_c64;
@@ -266,20 +267,11 @@ long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int s
     }
     b0 += g;
   }
-  if (!hip_ok(hipMemcpyAsync(idx, didx, cells * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream), "copy idx") ||
-      !hip_ok(hipMemcpyAsync(val_c64, dval, cells * sizeof(cf), hipMemcpyDeviceToHost, c->stream), "copy val") ||
-      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
-    return -1;
+  const long total = mfh::read_peaks(c, pb, cells, nb, idx, val_c64, counts);  // waits for the stream
+  if (total < 0) return -1;
   wait_on_return.armed = false;
   c->prof_collect();
   if (n_refined) *n_refined = refined_total;
-  long total = 0;
-  for (size_t p = 0; p < (size_t)nblocks * nt; p++) {
-    int cnt = 0;
-    for (size_t b = 0; b < nb; b++) cnt += idx[p * nb + b] >= 0;
-    if (counts) counts[p] = cnt;
-    total += cnt;
-  }
   return total;
 }
 

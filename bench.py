@@ -543,6 +543,14 @@ def fit_leg(args):
             "forward_ms": t_fwd * 1e3, "forward_backward_ms": t_both * 1e3}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(leg):
+    """One line per finished leg on stderr (the JSON result stays alone on stdout)."""
+    print("bench: %s done at %.1f s" % (leg, time.perf_counter() - _T0), file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -649,17 +657,17 @@ def main():
         out["roofline"]["frac_of_probed_fma"] = achieved / pr["fma_tflops"]
         out["roofline"]["frac_of_probed_add"] = achieved / pr["add_tflops"]
         args.copy_gbs = pr["copy_gbs"]
-    if not args.no_hier:
-        out["hierarchical"] = hier_leg(args)
-    if not args.no_large:
-        out["large_n"] = large_n_leg(args)
-        out["run_series"] = series_leg(args)
-    if not args.no_uneven:
-        out["uneven_refine"] = uneven_leg(args)
-    if not args.no_fit:
-        out["peak_fit"] = fit_leg(args)
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(D, T)
+    progress("headline")
+    legs = [("hierarchical", not args.no_hier, lambda: hier_leg(args)),
+            ("large_n", not args.no_large, lambda: large_n_leg(args)),
+            ("run_series", not args.no_large, lambda: series_leg(args)),
+            ("uneven_refine", not args.no_uneven, lambda: uneven_leg(args)),
+            ("peak_fit", not args.no_fit, lambda: fit_leg(args)),
+            ("cpu_baseline", not args.no_cpu, lambda: cpu_baseline(D, T))]
+    for name, on, leg in legs:
+        if on:
+            out[name] = leg()
+            progress(name)
     print(json.dumps(out), flush=True)
 
 

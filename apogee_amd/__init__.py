@@ -220,6 +220,8 @@ class MatchedFilter:
         """Launch shapes from the gfx950 autotuner table (defaults if none)."""
         from . import autotune
         autotune.apply(self._gpu, "flat", self.n, self.ndata, self.ntemplates)
+        if self.n >= 32768:
+            autotune.apply(self._gpu, "big", self.n, self.ndata, self.ntemplates)
 
     def _init_state(self):
         self._books = {"data": _RowBook(self.ndata), "template": _RowBook(self.ntemplates)}

@@ -5,6 +5,10 @@ Tunables (each a key of the library's tuning interface):
 * ``flat`` -- ``dpw``: data rows each workgroup of the tiled flat kernel
   walks while its template stays in registers (more rows: fewer template
   reads and fewer, longer workgroups; fewer rows: more workgroups in flight);
+* ``big`` -- ``big_chunk_mb``: for the four-step lengths (n >= 32768), the
+  pairs whose n-point intermediate is held at once (column launch, then row
+  launch, per chunk): larger chunks mean fewer, fuller launches until the
+  intermediate no longer stays in the memory-side cache between the two;
 * ``hier`` -- ``refine_grid``: workgroups of the list-driven refine launch,
   which drains however many pairs the coarse gate passed;
 * ``coarse`` -- ``coarse_lds`` x ``coarse_rpw`` x ``coarse_tpw``: where the
@@ -38,6 +42,7 @@ TABLE_PATH = os.path.join(_HERE, "tuning_gfx950.json")
 
 CANDIDATES = {
     "flat": {"dpw": [1, 2, 4, 8, 16]},
+    "big": {"big_chunk_mb": [64, 96, 128, 160, 192, 256]},
     "hier": {"refine_grid": [256, 512, 1024, 2048, 4096]},
     "coarse": {"coarse_lds": [0, 1], "coarse_rpw": [1, 2], "coarse_tpw": [1, 2, 4, 8, 16]},
     "queue": {"grid_per_cu": [1, 2]},
@@ -48,6 +53,7 @@ FLAT_SHAPES = [
     (1024, 16, 1024, 1024), (4096, 16, 1024, 4096), (4096, 64, 256, 512), (4096, 4, 4096, 4096),
     (16384, 16, 256, 16384), (65536, 4, 128, 1024),
 ]
+BIG_SHAPES = [(32768, 4, 256, 1024), (65536, 4, 128, 1024)]
 HIER_SHAPES = [(4096, 16, 1024, 4096, 512)]
 COARSE_SHAPES = [(4096, 16, 1024, 512), (4096, 64, 256, 512), (4096, 4, 4096, 512)]
 QUEUE_WORKLOADS = [("uneven-4000", 4000)]
@@ -135,6 +141,29 @@ def _tune_flat(device, n, nd, nt, binsize, reps, rng):
     ctx.destroy()
     return {"op": "flat", "n": n, "ndata": nd, "ntemplates": nt, "binsize": binsize,
             "best": {"dpw": best}, "ms": {str(k): round(v, 5) for k, v in ms.items()}}
+
+
+def _tune_big(device, n, nd, nt, binsize, reps, rng):
+    import numpy as np
+    from . import _hip
+    ctx = _hip.Context(device, n, nd, nt)
+    ctx.set_rows("data", 0, (rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))).astype(np.complex64))
+    ctx.set_rows("template", 0, (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64))
+    import torch
+    nb = -(-n // binsize)
+    idx = torch.empty(nd * nt * nb, dtype=torch.int64, device="cuda:%d" % device)
+    val = torch.empty(nd * nt * nb, dtype=torch.complex64, device="cuda:%d" % device)
+    ms = {}
+    for mb in CANDIDATES["big"]["big_chunk_mb"]:
+        ctx.set_tuning("big_chunk_mb", mb)
+        run = lambda: ctx.peaks_async(0, nd, 0, nt, binsize, 0.0, 0, n, idx.data_ptr(), val.data_ptr())
+        run()
+        ctx.sync()
+        ms[mb] = _median_ms(run, reps, ctx, "flat")
+    best = min(ms, key=ms.get)
+    ctx.destroy()
+    return {"op": "big", "n": n, "ndata": nd, "ntemplates": nt, "binsize": binsize,
+            "best": {"big_chunk_mb": best}, "ms": {str(k): round(v, 5) for k, v in ms.items()}}
 
 
 def _tune_hier(device, n, nd, nt, binsize, band, reps, rng):
@@ -324,7 +353,7 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=7)
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--only", default="",
-                    help="comma-separated operations to re-measure (flat, hier, coarse, band, queue); "
+                    help="comma-separated operations to re-measure (flat, big, hier, coarse, band, queue); "
                          "the other entries of the existing table are kept")
     args = ap.parse_args(argv)
     only = {o for o in args.only.split(",") if o}
@@ -347,6 +376,9 @@ def main(argv=None):
     if want("flat"):
         for n, nd, nt, bs in FLAT_SHAPES:
             add(_tune_flat(args.device, n, nd, nt, bs, args.reps, rng))
+    if want("big"):
+        for n, nd, nt, bs in BIG_SHAPES:
+            add(_tune_big(args.device, n, nd, nt, bs, args.reps, rng))
     if want("hier"):
         for n, nd, nt, bs, band in HIER_SHAPES:
             add(_tune_hier(args.device, n, nd, nt, bs, band, args.reps, rng))
@@ -359,7 +391,7 @@ def main(argv=None):
     if want("queue"):
         for wname, nreq in QUEUE_WORKLOADS:
             add(_tune_queue(args.device, wname, nreq, args.reps))
-    order = ["flat", "hier", "coarse", "band", "queue"]
+    order = ["flat", "big", "hier", "coarse", "band", "queue"]
     entries = sorted(keep + entries, key=lambda e: order.index(e["op"]) if e["op"] in order else 99)
     table = {"device": name, "arch": arch, "method": "median of %d event-timed launches" % args.reps,
              "entries": entries}

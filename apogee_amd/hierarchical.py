@@ -115,6 +115,8 @@ class HierarchicalFilter(MatchedFilter):
     def _apply_tuning(self):
         from . import autotune
         autotune.apply(self._gpu, "flat", self.n, self.ndata, self.ntemplates)
+        if self.n >= 32768:
+            autotune.apply(self._gpu, "big", self.n, self.ndata, self.ntemplates)
         autotune.apply(self._gpu, "hier", self.n, self.ndata, self.ntemplates)
         autotune.apply(self._gpu, "coarse", self.n, self.ndata, self.ntemplates)   # the band-512 MFMA screen
 

@@ -12,16 +12,17 @@ launch shape the autotuner may pick.
   coarse correlation within the screen's widening, the gate decisions a
   superset of FP64's, the one-sided guarantee against the device flat filter,
   and the injected detections kept.
-* Autotuner candidates: every flat dpw at every tuned shape, every refine
-  grid, and every persistent-queue grid (the MFMA rows x templates per wave
-  are covered in test_gpu_mfma.py).
+* Autotuner candidates: every flat dpw at every tuned shape, every
+  four-step chunk size at both four-step shapes, every refine grid, and
+  every persistent-queue grid (the MFMA rows x templates per wave are
+  covered in test_gpu_mfma.py).
 """
 import numpy as np
 import pytest
 
 import apogee_amd as mf
 import bench
-from apogee_amd.autotune import CANDIDATES, FLAT_SHAPES
+from apogee_amd.autotune import BIG_SHAPES, CANDIDATES, FLAT_SHAPES
 from apogee_amd.hierarchical import _COARSE_PRECISIONS
 from oracle import mf_oracle as O
 from _parity import assert_peaks_match, tol
@@ -92,6 +93,36 @@ def test_every_tuned_flat_shape_and_dpw(shape, gpu_device):
         else:
             np.testing.assert_array_equal(got["index"], base["index"], err_msg="dpw %d" % dpw)
             np.testing.assert_array_equal(got["value"], base["value"], err_msg="dpw %d" % dpw)
+
+
+@pytest.mark.parametrize("shape", BIG_SHAPES, ids=lambda s: "n%d_%dx%d_bs%d" % s)
+def test_every_four_step_chunk_is_bit_identical(shape, gpu_device):
+    """Each four-step shape of the tuning table at every chunk size (the
+    pairs whose intermediate is held at once: one chunk, or several with a
+    ragged last one): identical bits, and the FP64 oracle on a slice, with
+    a threshold so that the row kernel's below-seed waves skip selection."""
+    n, nd, nt, bs = shape
+    rng = np.random.default_rng(n + nt)
+    D = (rng.standard_normal((nd, n)) + 1j * rng.standard_normal((nd, n))).astype(np.complex64)
+    T = (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64)
+    T /= np.sqrt(np.sum(np.abs(T) ** 2, axis=1, keepdims=True))   # Re z, Im z of unit variance
+    D[1] += 8.0 * T[5]                                               # |z| = 8 at lag 0 of pair (1, 5)
+    thr = 4.0   # |z|^2 > 16 in noise: about 3e-4 per lag, so some bins are set and most waves skip
+    base = None
+    for mb in CANDIDATES["big"]["big_chunk_mb"]:
+        f = mf.MatchedFilter(n, nd, nt, device="gpu")
+        f._gpu.set_tuning("big_chunk_mb", mb)
+        f.set_data(D)
+        f.set_templates(T)
+        got = f.run(binsize=bs, threshold=thr).copy()
+        if base is None:
+            base = got
+            assert (got["index"] >= 0).any() and (got["index"] < 0).any()
+            assert_peaks_match(D, T[:8], got["index"][:, :8], got["value"][:, :8], bs, thr,
+                               context="shape %s" % (shape,))
+        else:
+            np.testing.assert_array_equal(got["index"], base["index"], err_msg="chunk %d MiB" % mb)
+            np.testing.assert_array_equal(got["value"], base["value"], err_msg="chunk %d MiB" % mb)
 
 
 # ------------------------------------------------------------ configuration 3

@@ -71,6 +71,31 @@ __device__ __forceinline__ cf big_root(const BigArgs &a, int m) { return cmul(a.
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
+// One (pair, bin) cell of a chunk whose rows launch left per-part tables:
+// the largest key over the parts (keys are distinct per lag, so exactly one
+// part holds it) and that part's value.  The keys are read first, all
+// independent loads, then the one value.
+__device__ __forceinline__ void finish_cell(const BigArgs &a, long long np, long long i) {
+  const int nb = (int)a.nb;
+  if (i >= np * nb) return;
+  const long long pl = i / nb;
+  const int b = (int)(i % nb);
+  int d = 0, tt = 0;
+  if (!big_pair(a, a.p0 + pl, d, tt)) return;
+  const uint64_t *keys = a.pkey + pl * a.nsplit * (long long)nb + b;
+  uint64_t best = 0ull;
+  int bw = 0;
+#pragma unroll 8
+  for (int w = 0; w < a.nsplit; w++) {
+    const uint64_t key = keys[(long long)w * nb];
+    bw = key > best ? w : bw;
+    best = key > best ? key : best;
+  }
+  const long long o = a.list_ooff ? a.list_ooff[a.p0 + pl] : ((long long)d * a.nt + tt) * a.ostride;
+  a.out_idx[o + b] = best ? (int64_t)(~(uint32_t)(best & 0xffffffffull)) : -1;
+  a.out_val[o + b] = best ? a.pval[(pl * a.nsplit + bw) * (long long)nb + b] : cf{0.f, 0.f};
+}
+
 // Columns: 16 columns f1 of one pair per workgroup; lane = 16 * t + s, so the
 // 16 lanes of a row group touch 16 consecutive columns (128-byte accesses).
 // The pair's rows and its Y block are wave-uniform, so loads and stores go
@@ -123,12 +148,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
   }
 }
 
-// Rows: nsplit workgroups per pair (blockIdx.y = part), each walking its
-// share of the 256 rows k2, SLOTS rows at a time.  With one part the table is
-// the result; with several, each part leaves its (key, value) table for
-// big_rows_finish_kernel, which keeps the largest key per bin.
+// Rows: nsplit workgroups per pair (workgroup = pair * nsplit + part), each
+// walking its share of the 256 rows k2, SLOTS rows at a time.  With one part
+// the table is the result; with several, each part leaves its (key, value)
+// table, and finish_cell keeps the largest key per bin: in the next chunk's
+// rows launch, or in big_rows_finish_kernel after the last chunk.
 template <int N1>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) big_rows_kernel(BigArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+big_rows_kernel(BigArgs a, BigArgs fin, long long fin_np, int fin_blocks) {
+  // the last fin_blocks workgroups finish the previous chunk (its tables
+  // were completed by the previous launch): one thread per (pair, bin)
+  const int row_blocks = (int)gridDim.x - fin_blocks;
+  if ((int)blockIdx.x >= row_blocks) {
+    finish_cell(fin, fin_np, (long long)(blockIdx.x - row_blocks) * 256 + threadIdx.x);
+    return;
+  }
   using S = FlatShape<N1>;
   constexpr int TPT = S::TPT;
   constexpr int SLOTS = S::SLOTS;
@@ -139,7 +173,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
   const int slot = threadIdx.x / TPT;
   const int t = threadIdx.x % TPT;
   cf *xch = xch_all + slot * S::XSTRIDE;
-  const long long pl = blockIdx.x;
+  const long long pl = blockIdx.x / a.nsplit;  // pair within the chunk
+  const int part = (int)(blockIdx.x % a.nsplit);
   int d = 0, tt = 0;
   if (!big_pair(a, a.p0 + pl, d, tt)) return;  // uniform; before any barrier
   const int nb = (int)a.nb;
@@ -147,7 +182,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
   __syncthreads();
   const cf *Y = a.scratch + pl * (long long)(N1 * 256);
   constexpr int G = 256 / SLOTS;  // row groups of the pair
-  const int gbeg = (int)blockIdx.y * G / a.nsplit, gend = ((int)blockIdx.y + 1) * G / a.nsplit;
+  const int gbeg = part * G / a.nsplit, gend = (part + 1) * G / a.nsplit;
   const bool whole = a.ws == 0 && a.we == (uint32_t)(N1 * 256);
   const bool full8 = whole && a.bshift == ilog2(N1 * 256 / 8);
   FlatArgs fa;  // for bin_of / make_key
@@ -229,11 +264,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
     __syncthreads();  // values settled before the next group raises keys; xch reusable
   }
   if (a.nsplit > 1) {
-    const long long part = (pl * a.nsplit + blockIdx.y) * (long long)nb;
+    const long long po = (pl * a.nsplit + part) * (long long)nb;
     for (int b = threadIdx.x; b < nb; b += 256) {
       const uint64_t key = bkey[b];
-      a.pkey[part + b] = key;
-      a.pval[part + b] = key ? bval[b] : cf{0.f, 0.f};
+      a.pkey[po + b] = key;
+      a.pval[po + b] = key ? bval[b] : cf{0.f, 0.f};
     }
     return;
   }
@@ -252,32 +287,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) b
   }
 }
 
-// One thread per (pair, bin) of a chunk: the largest key over the parts
-// (keys are distinct per lag, so exactly one part holds it) and its value.
-// (Merging in the rows kernel instead -- the last part of a pair to finish,
-// found by a ticket counter -- needs device-scope fences around the ticket,
-// and measured 3.5x slower for the whole rows launch on gfx950.)
+// The last chunk's finish (every other chunk's runs inside the next rows
+// launch): one thread per (pair, bin) cell.
 __global__ void __launch_bounds__(256) big_rows_finish_kernel(BigArgs a, long long np) {
-  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-  const int nb = (int)a.nb;
-  if (i >= np * nb) return;
-  const long long pl = i / nb;
-  const int b = (int)(i % nb);
-  int d = 0, tt = 0;
-  if (!big_pair(a, a.p0 + pl, d, tt)) return;
-  uint64_t best = 0ull;
-  cf bv = cf{0.f, 0.f};
-  for (int w = 0; w < a.nsplit; w++) {
-    const long long part = (pl * a.nsplit + w) * (long long)nb + b;
-    const uint64_t key = a.pkey[part];
-    if (key > best) {
-      best = key;
-      bv = a.pval[part];
-    }
-  }
-  const long long o = a.list_ooff ? a.list_ooff[a.p0 + pl] : ((long long)d * a.nt + tt) * a.ostride;
-  a.out_idx[o + b] = best ? (int64_t)(~(uint32_t)(best & 0xffffffffull)) : -1;
-  a.out_val[o + b] = best ? bv : cf{0.f, 0.f};
+  finish_cell(a, np, (long long)blockIdx.x * 256 + threadIdx.x);
 }
 
 __global__ void roots_kernel(cf *out, int L, int period) {
@@ -347,15 +360,31 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
   a.nsplit = nsplit;
   a.pkey = nullptr;
   a.pval = nullptr;
+  // part tables in two halves, by chunk parity: a rows launch writes one
+  // while its finishing blocks read the previous chunk's
+  const size_t part_cells = (size_t)chunk * nsplit * nb_piece;
+  uint64_t *pkey2 = nullptr;
+  cf *pval2 = nullptr;
   if (nsplit > 1) {
-    a.pkey = (uint64_t *)c->scratch("big_part_key", (size_t)chunk * nsplit * nb_piece * sizeof(uint64_t));
-    a.pval = (cf *)c->scratch("big_part_val", (size_t)chunk * nsplit * nb_piece * sizeof(cf));
-    if (!a.pkey || !a.pval) return -1;
+    pkey2 = (uint64_t *)c->scratch("big_part_key", 2 * part_cells * sizeof(uint64_t));
+    pval2 = (cf *)c->scratch("big_part_val", 2 * part_cells * sizeof(cf));
+    if (!pkey2 || !pval2) return -1;
   }
+  // the finish of a chunk runs in the next rows launch when the window is
+  // one piece (one rows launch per chunk); else after each rows launch
+  const bool fold = nsplit > 1 && nb_total <= maxb;
+  BigArgs fin = a;
+  long long fin_np = 0;
+  int fin_blocks = 0;
+  long long ci = 0;  // chunk index
   Ctx::ProfScope ps(c, r.list ? r.list_prof_name : r.prof_name);
-  for (long long p0 = 0; p0 < total; p0 += chunk) {
+  for (long long p0 = 0; p0 < total; p0 += chunk, ci++) {
     const long long np = std::min(chunk, total - p0);
     a.p0 = p0;
+    if (nsplit > 1) {
+      a.pkey = pkey2 + (ci & 1) * part_cells;
+      a.pval = pval2 + (ci & 1) * part_cells;
+    }
     hipLaunchKernelGGL(big_cols_kernel<N1>, dim3(N1 / 16, (unsigned)np), dim3(256), lds_cols, c->stream, a);
     if (!hip_ok(hipGetLastError(), "big_cols_kernel")) return -1;
     for (size_t b0 = 0; b0 < nb_total; b0 += maxb) {
@@ -367,15 +396,28 @@ int launch_big_n1(Ctx *c, const FlatRun &r) {
       a.out_idx = r.out_idx + b0;
       a.out_val = r.out_val + b0;
       const size_t lds = (size_t)S::SLOTS * S::XSTRIDE * sizeof(cf) + cnt * (sizeof(uint64_t) + sizeof(cf));
-      hipLaunchKernelGGL(big_rows_kernel<N1>, dim3((unsigned)np, (unsigned)nsplit), dim3(256), lds, c->stream, a);
+      const long long row_blocks = np * nsplit;
+      hipLaunchKernelGGL(big_rows_kernel<N1>, dim3((unsigned)(row_blocks + fin_blocks)), dim3(256), lds, c->stream, a,
+                         fin, fin_np, fin_blocks);
       if (!hip_ok(hipGetLastError(), "big_rows_kernel")) return -1;
+      fin_blocks = 0;
       if (nsplit > 1) {
         const long long cells = np * (long long)cnt;
-        hipLaunchKernelGGL(big_rows_finish_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, c->stream, a,
-                           np);
-        if (!hip_ok(hipGetLastError(), "big_rows_finish_kernel")) return -1;
+        if (fold) {  // finished by the next rows launch, or after the loop
+          fin = a;
+          fin_np = np;
+          fin_blocks = (int)((cells + 255) / 256);
+        } else {
+          hipLaunchKernelGGL(big_rows_finish_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, c->stream,
+                             a, np);
+          if (!hip_ok(hipGetLastError(), "big_rows_finish_kernel")) return -1;
+        }
       }
     }
+  }
+  if (fin_blocks) {
+    hipLaunchKernelGGL(big_rows_finish_kernel, dim3((unsigned)fin_blocks), dim3(256), 0, c->stream, fin, fin_np);
+    if (!hip_ok(hipGetLastError(), "big_rows_finish_kernel")) return -1;
   }
   return 0;
 }

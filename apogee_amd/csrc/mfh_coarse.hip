@@ -336,37 +336,61 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
   // LDSC: the workgroup's templates' fragments, [template][quad][lane] in
   // 16-byte units (conflict-free reads), staged once for its four waves
   extern __shared__ uint4 tfrag[];
+  const int d0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * R;
+  const int q = lane >> 4, r = lane & 15;
+  // Every global load of the prologue is issued before anything waits on
+  // one: the data rows, then (LDSC) the constants and the template
+  // fragments, so the workgroup pays one memory latency, not one per loop
+  // trip.  The data rows at the A-fragment positions f = 128 q + 16 i + r; a
+  // row past nd (the wave's last, when nd is not a multiple of R) is read
+  // from row nd - 1 and zeroed below.
+  cf x[R][8];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const cf *D = data + (long long)min(d0 + k, nd - 1) * data_stride;
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[k][i] = D[128 * q + 16 * i + r];
+  }
   if constexpr (LDSC) {
-    // global table: lane-major, 12 words4 per lane; LDS: word4-major
+    // global table: lane-major, 12 words4 per lane (768 = 3 per thread);
+    // LDS: word4-major
     const uint4 *src = reinterpret_cast<const uint4 *>(consts);
-    for (int i = threadIdx.x; i < 12 * 64; i += 256) ctab[(i % 12) * 64 + i / 12] = src[i];
-    const int tw0 = blockIdx.x * tpw, tcnt = min(nt, tw0 + tpw) - tw0;
-    const uint4 *tsrc = reinterpret_cast<const uint4 *>(ctm) + (long long)tw0 * 256;
-    for (int i = threadIdx.x; i < tcnt * 256; i += 256) {
-      const int j = i >> 8, l = (i >> 2) & 63, q4 = i & 3;
-      tfrag[(j * 4 + q4) * 64 + l] = tsrc[i];
+    const uint4 cw0 = src[threadIdx.x], cw1 = src[threadIdx.x + 256], cw2 = src[threadIdx.x + 512];
+    const int tw0 = blockIdx.x * tpw, tcnt = min(nt, tw0 + tpw) - tw0;  // >= 1
+    const uint4 *tsrc = reinterpret_cast<const uint4 *>(ctm) + (long long)tw0 * 256 + threadIdx.x;
+    uint4 *tdst = tfrag + ((threadIdx.x & 3) * 64 + ((threadIdx.x >> 2) & 63));  // word4 256 j + threadIdx.x
+    // four templates' words per trip, read from a clamped (valid) template
+    // and stored only where the template exists
+    for (int j0 = 0; j0 < tcnt; j0 += 4) {
+      const int jl = tcnt - 1;
+      const uint4 v0 = tsrc[min(j0, jl) * 256], v1 = tsrc[min(j0 + 1, jl) * 256];
+      const uint4 v2 = tsrc[min(j0 + 2, jl) * 256], v3 = tsrc[min(j0 + 3, jl) * 256];
+      if (j0 == 0) {
+        ctab[(threadIdx.x % 12) * 64 + threadIdx.x / 12] = cw0;
+        ctab[((threadIdx.x + 256) % 12) * 64 + (threadIdx.x + 256) / 12] = cw1;
+        ctab[((threadIdx.x + 512) % 12) * 64 + (threadIdx.x + 512) / 12] = cw2;
+      }
+      tdst[j0 * 256] = v0;
+      if (j0 + 1 < tcnt) tdst[(j0 + 1) * 256] = v1;
+      if (j0 + 2 < tcnt) tdst[(j0 + 2) * 256] = v2;
+      if (j0 + 3 < tcnt) tdst[(j0 + 3) * 256] = v3;
     }
     __syncthreads();
   }
-  const int d0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * R;
   if (g.zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *g.zero = 0;
   // a wave past the last data row walks no templates but stays for the
   // workgroup's barriers at the end
   const int t0 = blockIdx.x * tpw;
   const int t1 = d0 < nd ? min(nt, t0 + tpw) : t0;
-  const int q = lane >> 4, r = lane & 15;
 
-  // the data rows at the A-fragment positions f = 128 q + 16 i + r; a row
-  // past nd (the wave's last, when nd is not a multiple of R) stays zero
-  cf x[R][8];
+  // each row scaled by an exact power of two to components below 1/2
   int sd[R];
 #pragma unroll
   for (int k = 0; k < R; k++) {
     float dmax = 0.f;
-    const cf *D = data + (long long)min(d0 + k, nd - 1) * data_stride;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
-      x[k][i] = d0 + k < nd ? D[128 * q + 16 * i + r] : cf{0.f, 0.f};
+      if (d0 + k >= nd) x[k][i] = cf{0.f, 0.f};
       dmax = fmaxf(dmax, fmaxf(fabsf(x[k][i].x), fabsf(x[k][i].y)));
     }
     sd[k] = unit_exponent(wave_max_nonneg(dmax)) + 1;  // components below 1/2

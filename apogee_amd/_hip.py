@@ -77,6 +77,7 @@ def _declare(lib):
     lib.mfh_probe_valu.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.mfh_probe_copy.argtypes = [ctypes.c_int, _c_size, ctypes.POINTER(ctypes.c_double)]
+    lib.mfh_stream_wait.argtypes = [_vp, _vp]
     lib.mfh_set_tuning.argtypes = [_vp, ctypes.c_char_p, ctypes.c_long]
     lib.mfh_prof_enable.argtypes = [_vp, ctypes.c_int]
     lib.mfh_prof_read.argtypes = [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
@@ -246,6 +247,14 @@ class DeviceView:
         self.compact, self.device, self.capsule = compact, device, capsule
 
 
+def _torch_stream(torch, device):
+    """PyTorch's current stream of ``device`` as a raw hipStream_t value."""
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is not None:
+        return int(raw(device))
+    return int(torch.cuda.current_stream(device).cuda_stream)
+
+
 def device_view(producer, stream):
     """Borrow ``producer``'s ROCm buffer for work queued on ``stream``.
 
@@ -356,7 +365,27 @@ class Context:
         self._bound[kind] = view
 
     def borrow(self, producer):
-        """:func:`device_view` of ``producer`` ordered on this context's stream."""
+        """:func:`device_view` of ``producer`` ordered on this context's stream.
+
+        A PyTorch tensor takes the short path of the same hand-over: the
+        context's stream waits on an event recorded on PyTorch's current
+        stream of the tensor's device (``mfh_stream_wait``), and the view
+        keeps the tensor alive -- what ``__dlpack__(stream=...)`` does,
+        without building and parsing a capsule on every call.
+        """
+        try:
+            import torch
+        except ImportError:
+            torch = None
+        if torch is not None and isinstance(producer, torch.Tensor) and producer.is_cuda:
+            t = producer.detach() if producer.requires_grad else producer
+            if t.is_conj():
+                t = t.resolve_conj()
+            dev = t.get_device()
+            check(lib().mfh_stream_wait(self.handle, ctypes.c_void_p(_torch_stream(torch, dev))),
+                  "mfh_stream_wait")
+            return DeviceView(int(t.data_ptr()), tuple(t.shape), t.dtype == torch.complex64,
+                              t.is_contiguous(), dev, t)
         return device_view(producer, self.stream())
 
     def bind_tensor(self, kind, tensor):

@@ -62,6 +62,7 @@ Ctx::~Ctx() {
     hipEventDestroy(e.b);
   }
   for (auto e : event_pool) hipEventDestroy(e);
+  if (order_ev) hipEventDestroy(order_ev);
   if (stream) hipStreamDestroy(stream);
   if (stage) hipHostFree(stage);
 }
@@ -280,6 +281,20 @@ extern "C" int mfh_sync(mfh_ctx *ctx) {
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   hipSetDevice(c->dev);
   return hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize") ? 0 : -1;
+}
+
+extern "C" int mfh_stream_wait(mfh_ctx *ctx, void *producer_stream) {
+  if (!ctx) return fail("null context");
+  Ctx *c = reinterpret_cast<Ctx *>(ctx);
+  hipSetDevice(c->dev);
+  if (!c->order_ev && !hip_ok(hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming), "hipEventCreate"))
+    return -1;
+  // a wait enqueued here holds the event's state at this call, so the event
+  // can be recorded again by the next hand-over
+  return hip_ok(hipEventRecord(c->order_ev, (hipStream_t)producer_stream), "hipEventRecord") &&
+                 hip_ok(hipStreamWaitEvent(c->stream, c->order_ev, 0), "hipStreamWaitEvent")
+             ? 0
+             : -1;
 }
 
 // Rows [row0, row0 + rows) of the data (templates = false) or template bank

@@ -23,6 +23,7 @@ struct Ctx {
   size_t n = 0;
   int nd = 0, nt = 0;
   hipStream_t stream = nullptr;
+  hipEvent_t order_ev = nullptr;  // mfh_stream_wait's hand-over event (created on first use)
   // The banks every launch reads: [nd][n] and [nt][n].  Each points either at
   // the context's own allocation (data_own / tmpl_own) or, after
   // mfh_bind_data / mfh_bind_templates, at a caller's device buffer that is

@@ -64,6 +64,15 @@ void *mfh_stream(mfh_ctx *ctx);
 /* Wait for all work queued on the context's stream. */
 int mfh_sync(mfh_ctx *ctx);
 
+/* Order the context's stream after the work queued so far on
+ * `producer_stream` (a hipStream_t of the context's device; null = the null
+ * stream): an event recorded there that the context's stream waits on, with
+ * no host wait.  The Python binding hands over PyTorch tensors this way,
+ * which is what a DLPack __dlpack__(stream=...) exchange does, without the
+ * capsule.  No counterpart in the reference, which refuses device arrays
+ * (python/matchedfilter/__init__.py:103-131). */
+int mfh_stream_wait(mfh_ctx *ctx, void *producer_stream);
+
 /* Copy `rows` spectra (rows x n complex64, host memory) into data rows
  * [row0, row0 + rows).  The library copies; the caller's buffer is free on
  * return.

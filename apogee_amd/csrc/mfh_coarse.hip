@@ -333,6 +333,7 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
                                                           const uint32_t *consts, float *cmax2, CoarseGate g) {
   const int lane = threadIdx.x & 63;
   __shared__ uint4 ctab[LDSC ? 12 * 64 : 1];
+  __shared__ int tsexp[LDSC ? 64 : 1];  // LDSC: the workgroup's templates' scale exponents (tpw <= 64)
   // LDSC: the workgroup's templates' fragments, [template][quad][lane] in
   // 16-byte units (conflict-free reads), staged once for its four waves
   extern __shared__ uint4 tfrag[];
@@ -357,6 +358,7 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
     const uint4 *src = reinterpret_cast<const uint4 *>(consts);
     const uint4 cw0 = src[threadIdx.x], cw1 = src[threadIdx.x + 256], cw2 = src[threadIdx.x + 512];
     const int tw0 = blockIdx.x * tpw, tcnt = min(nt, tw0 + tpw) - tw0;  // >= 1
+    const int se = (int)threadIdx.x < tcnt ? sexp[tw0 + threadIdx.x] : 0;
     const uint4 *tsrc = reinterpret_cast<const uint4 *>(ctm) + (long long)tw0 * 256 + threadIdx.x;
     uint4 *tdst = tfrag + ((threadIdx.x & 3) * 64 + ((threadIdx.x >> 2) & 63));  // word4 256 j + threadIdx.x
     // four templates' words per trip, read from a clamped (valid) template
@@ -366,6 +368,7 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
       const uint4 v0 = tsrc[min(j0, jl) * 256], v1 = tsrc[min(j0 + 1, jl) * 256];
       const uint4 v2 = tsrc[min(j0 + 2, jl) * 256], v3 = tsrc[min(j0 + 3, jl) * 256];
       if (j0 == 0) {
+        if ((int)threadIdx.x < tcnt) tsexp[threadIdx.x] = se;
         ctab[(threadIdx.x % 12) * 64 + threadIdx.x / 12] = cw0;
         ctab[((threadIdx.x + 256) % 12) * 64 + (threadIdx.x + 256) / 12] = cw1;
         ctab[((threadIdx.x + 512) % 12) * 64 + (threadIdx.x + 512) / 12] = cw2;
@@ -416,7 +419,9 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
       else
         lm[k] = dft.template lane_max<FULL>(x[k], ct, cs, ce, q, r);
     }
-    const int st = sexp[t];
+    // the template's scale exponent: from LDS (staged with the fragments)
+    // rather than a global load whose latency the pair chain would wait on
+    const int st = LDSC ? tsexp[t - t0] : sexp[t];
 #pragma unroll
     for (int k = 0; k < R; k++) {
       const float m2 = ldexpf(__uint_as_float(wave_umax(lm[k])), 2 * (st + sd[k]));

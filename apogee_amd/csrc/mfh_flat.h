@@ -463,6 +463,10 @@ flat_kernel(FlatArgs a) {
   __syncthreads();
 
   const int dstep = LIST ? (int)gridDim.x * S::SLOTS : 1;
+  // list mode: each pair's list entry is read one trip ahead, so the load
+  // that the row addresses depend on is in flight during the previous pair
+  int2 pr_next = int2{0, 0};
+  if constexpr (LIST) pr_next = dbeg + slot < dend ? a.list[dbeg + slot] : int2{0, 0};
   for (int dd = dbeg, it = 0; dd < dend; dd += dstep, it++) {
     // Everything derived from the thread index (bins, window flags, LDS and
     // twiddle addresses) is loop-invariant; recomputing it per pair is far
@@ -476,7 +480,8 @@ flat_kernel(FlatArgs a) {
     if constexpr (LIST) {
       const int li = dd + slot;
       valid = li < dend;
-      const int2 pr = valid ? a.list[li] : int2{0, 0};
+      const int2 pr = pr_next;  // a.list[li] when valid
+      pr_next = li + dstep < dend ? a.list[li + dstep] : int2{0, 0};
       d = pr.x;
       tt = pr.y;
       const cf *dp = a.data + (long long)d * a.data_stride;

@@ -65,6 +65,7 @@ Ctx::~Ctx() {
   if (order_ev) hipEventDestroy(order_ev);
   if (stream) hipStreamDestroy(stream);
   if (stage) hipHostFree(stage);
+  if (refine_hint) hipHostFree(refine_hint);
 }
 
 void *Ctx::host_stage(size_t bytes) {

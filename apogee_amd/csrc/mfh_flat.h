@@ -49,6 +49,7 @@ struct FlatArgs {
   float inv_bs;        // 1 / binsize
   float seed;          // threshold^2 (FP32), or 0
   int *zero;           // if set, workgroup 0 stores 0 there (a counter for the next kernel)
+  int *hint;           // list mode, if set: workgroup 0 stores the list length there (host-mapped)
 };
 
 template <int N>
@@ -451,13 +452,17 @@ flat_kernel(FlatArgs a) {
     dbeg = blockIdx.x * S::SLOTS;
     dend = *a.list_count;
   }
+  if (a.zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *a.zero = 0;
+  if constexpr (LIST) {
+    // the list length, for the host's sizing of the next refine launch
+    if (a.hint && blockIdx.x == 0 && threadIdx.x == 0) *a.hint = dend;
+    // workgroup-uniform: the list is shorter than this slot range (with few
+    // survivors most of a refine grid leaves here, before any other load)
+    if (dbeg >= dend) return;
+  }
   // the first twiddled pass's twiddles, once for every pair this thread runs
   cf wfirst[15];
   first_twiddles<N>(wfirst, t0, a.twiddle);
-  if (a.zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *a.zero = 0;
-  if constexpr (LIST) {
-    if (dbeg >= dend) return;  // workgroup-uniform: the list is shorter than this slot range
-  }
   // both bin buffers start empty
   for (int i = threadIdx.x; i < 2 * S::SLOTS * nb; i += S::WG) bins_all[i] = 0ull;
   __syncthreads();

@@ -67,14 +67,23 @@ int launch_n(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, si
   a.list_count = r.list_count;
   a.list_ooff = r.list_ooff;
   a.zero = bin0 == 0 ? r.zero : nullptr;
+  a.hint = bin0 == 0 ? r.hint : nullptr;
   const bool list = r.list != nullptr;
   int dpw = c->tune_dpw;
   if (dpw <= 0) dpw = std::min(r.nd, 4);
   a.dpw = dpw;
   dim3 grid;
   if (list) {
-    // enough workgroups to fill the device; each drains the list grid-stride
-    const int want = (r.list_cap + S::SLOTS - 1) / S::SLOTS;
+    // enough workgroups to fill the device; each drains the list grid-stride.
+    // With a hint (the previous call's list length), about two pairs per
+    // workgroup and at least 512 workgroups: an emptier list leaves most of
+    // a full grid with nothing to do, and each idle workgroup still costs
+    // its dispatch (any grid drains any list, so a stale hint is harmless)
+    int want = (r.list_cap + S::SLOTS - 1) / S::SLOTS;
+    if (r.hint) {
+      const int last = *(volatile int *)r.hint;
+      want = std::min(want, std::max(512, (std::max(last, 0) / S::SLOTS + 1) / 2));
+    }
     grid = dim3((unsigned)std::max(1, std::min(want, c->refine_grid_cap)), 1);
   } else {
     grid = dim3((unsigned)((r.nt + S::SLOTS - 1) / S::SLOTS), (unsigned)((r.nd + dpw - 1) / dpw));

@@ -246,6 +246,14 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   rr.list = list;
   rr.list_count = count;
   rr.list_cap = (int)pairs;
+  if (!hs && !c->no_alloc && !c->capturing && !c->refine_hint) {
+    // one pinned word per context, mapped for the device (default flags);
+    // starts at the full list so the first call sizes as before
+    if (!hip_ok(hipHostMalloc((void **)&c->refine_hint, sizeof(int), hipHostMallocDefault), "hipHostMalloc"))
+      return -1;
+    *c->refine_hint = (int)pairs;
+  }
+  rr.hint = hs ? nullptr : c->refine_hint;
   if (launch_flat(c, n, rr) < 0) return -1;
   if (count_out &&
       !hip_ok(hipMemcpyAsync(count_out, count, sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy count"))

@@ -60,7 +60,10 @@ struct Ctx {
   // four-step lengths (32768, 65536): roots built, scratch budget per chunk
   bool big_roots_ready[16] = {};  // by log2 of the column length
   long long big_chunk_bytes = 128ll << 20;  // measured best on gfx950 (256: +7%, 64: +16%)
-  int refine_grid_cap = 2048;  // workgroups of a list-driven (refine) launch
+  int refine_grid_cap = 2048;  // workgroups of a list-driven (refine) launch, at most
+  // pinned host word holding the last hierarchical refine's survivor count,
+  // written by that launch (a sizing hint only: results never depend on it)
+  int *refine_hint = nullptr;
   // Per-pass twiddle tables of the flat kernel for length n (FP64-generated
   // on first use, kept for the context's lifetime).
   std::map<size_t, cf *> tw_by_n;
@@ -159,6 +162,9 @@ struct FlatRun {
   const char *prof_name = "flat";        // event-timing name of a tiled launch
   const char *list_prof_name = "refine";  // and of a list-mode launch
   int *zero = nullptr;                    // if set, the launch stores 0 there
+  // list mode: a host-mapped word the launch stores the list length in, and
+  // from which the launch's grid is sized (the previous call's length)
+  int *hint = nullptr;
 };
 
 bool flat_supported(size_t n);

@@ -332,6 +332,11 @@ class Context:
     def sync(self):
         check(lib().mfh_sync(self.handle), "mfh_sync")
 
+    def wait_for(self, stream):
+        """Order this context's stream after the work queued on ``stream``
+        (a raw hipStream_t value) so far, with no host wait."""
+        check(lib().mfh_stream_wait(self.handle, ctypes.c_void_p(int(stream))), "mfh_stream_wait")
+
     # -------------------------------------------------------------- ingest
     def set_rows(self, kind, row0, spectra):
         """Copy host spectra, a C-contiguous complex64 array (rows, n), into
@@ -382,8 +387,7 @@ class Context:
             if t.is_conj():
                 t = t.resolve_conj()
             dev = t.get_device()
-            check(lib().mfh_stream_wait(self.handle, ctypes.c_void_p(_torch_stream(torch, dev))),
-                  "mfh_stream_wait")
+            self.wait_for(_torch_stream(torch, dev))
             return DeviceView(int(t.data_ptr()), tuple(t.shape), t.dtype == torch.complex64,
                               t.is_contiguous(), dev, t)
         return device_view(producer, self.stream())

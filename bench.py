@@ -300,7 +300,10 @@ def hier_leg(args):
                        "at SNR 8; false dismissal over %d single SNR-8 injections per screen, and "
                        "over as many at the calibration point SNR 5.5" % args.fd_injections,
            "flat_pairs_per_s": pairs / t_flat, "flat_kernel_us": st_f["flat"],
-           "flat_pairs_per_s_injected": pairs / t_flat_inj, "screens": {}}
+           "flat_pairs_per_s_injected": pairs / t_flat_inj,
+           # pairs whose flat peak is above the threshold on the injected
+           # batch: every one of them must be refined by an exact screen
+           "flat_pairs_above_threshold_injected": int((flat_inj >= 0).sum()), "screens": {}}
     for k, prec in enumerate(("fp16", "bf16", "fp32")):
         hier = mf.HierarchicalFilter(N, ND, NT, snr=5.5, fd=1e-2, band=512, device="gpu:0")
         hier.set_reference(power)

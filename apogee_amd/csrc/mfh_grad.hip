@@ -55,30 +55,38 @@ __device__ __forceinline__ cf circle(const cf *lds, int n, uint32_t m) {
   return cmul(lds[m >> 8], lds[nh + (m & 255u)]);
 }
 
-// dL/dT (TEMPL = true) or dL/dD: one workgroup per (row, 64 consecutive f);
+// dL/dT (TEMPL = true) or dL/dD: one workgroup per (row, 64 F consecutive f);
 // the row is blockIdx.y, the other side's rows are summed over.  The row's
 // (lag, gradient) list is staged through LDS in chunks, so the inner loop
-// reads LDS instead of waiting on one global load per term, and the four
-// waves of the workgroup take every fourth entry of a chunk: four times the
-// waves in flight of one thread per (row, f), then a fixed-order sum of the
-// four partial sums (deterministic).
-template <bool TEMPL>
-__global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ other, long long other_stride,
+// reads LDS instead of waiting on one global load per term.  The eight
+// waves take the other side's rows in turn (row j to wave j mod 8), so each
+// reads its row's F spectrum values once for all of that row's bins; a
+// fixed-order sum of the eight partial sums keeps the result deterministic.
+// Each lane covers F consecutive f: w^(f k) for the first comes from the
+// tables (one scattered lookup), the rest by F - 1 products with w^k (one
+// lookup at a uniform address per term), so scattered LDS reads -- whose
+// bank conflicts bound this kernel -- drop F-fold.
+template <bool TEMPL, int F>
+__global__ void __launch_bounds__(512) peak_grad_kernel(const cf *__restrict__ other, long long other_stride,
                                                         int nother, const int64_t *__restrict__ idx,
                                                         const cf *__restrict__ gval, int nd, int nt, int nb,
                                                         const cf *__restrict__ tables, int n, cf *__restrict__ out,
                                                         long long out_stride) {
   constexpr int CH = 1024;  // (lag, gradient) entries per LDS chunk
+  constexpr int W = 8;      // waves
   __shared__ cf roots[512];
   __shared__ uint32_t sk[CH];
   __shared__ cf sg[CH];
-  __shared__ cf part[4][64];
+  __shared__ cf part[W][64 * F];
   stage_circle(tables, n, roots);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int f = blockIdx.x * 64 + lane;  // n is a multiple of 64
+  const int fb = blockIdx.x * 64 * F;  // n is a multiple of 64 F
+  const int f0 = fb + lane * F;
   const int row = blockIdx.y;
   const uint32_t mask = (uint32_t)n - 1u;
-  float ax = 0.f, ay = 0.f;
+  float ax[F], ay[F];
+#pragma unroll
+  for (int q = 0; q < F; q++) ax[q] = ay[q] = 0.f;
   const long long total = (long long)nother * nb;
   for (long long e0 = 0; e0 < total; e0 += CH) {
     const int cnt = (int)(total - e0 < CH ? total - e0 : CH);
@@ -92,37 +100,50 @@ __global__ void __launch_bounds__(256) peak_grad_kernel(const cf *__restrict__ o
       sg[i] = gval[pair * nb + b];
     }
     __syncthreads();
-    int jx = -1;
-    cf x = cf{0.f, 0.f};
-    for (int i = w; i < cnt; i += 4) {
-      const uint32_t k = sk[i];
-      if (k != 0xffffffffu) {
-        const int j = (int)((e0 + i) / nb);
-        if (j != jx) {
-          x = other[(long long)j * other_stride + f];
-          jx = j;
-        }
+    // this wave's rows of the chunk: j = w (mod W)
+    const long long j0 = e0 / nb, j1 = (e0 + cnt - 1) / nb;
+    for (long long j = j0 + ((w - j0 % W) + W) % W; j <= j1; j += W) {
+      const int ib = (int)(j * nb - e0 > 0 ? j * nb - e0 : 0);
+      const int ie = (int)((j + 1) * nb - e0 < cnt ? (j + 1) * nb - e0 : cnt);
+      cf x[F];
+      const cf *xr = other + j * other_stride + f0;
+#pragma unroll
+      for (int q = 0; q < F; q++) x[q] = xr[q];
+      for (int i = ib; i < ie; i++) {
+        const uint32_t k = sk[i];
+        if (k == 0xffffffffu) continue;  // uniform: one term for the whole wave
         const cf g = sg[i];
-        const cf wk = circle(roots, n, ((uint32_t)f * k) & mask);
-        // templates: conj(g) x w ; data: g x conj(w)
-        cf xw;
-        if (TEMPL) {
-          xw = cmul(x, wk);
-          ax += g.x * xw.x + g.y * xw.y;
-          ay += g.x * xw.y - g.y * xw.x;
-        } else {
-          xw = cmul_conj(x, wk);
-          ax += g.x * xw.x - g.y * xw.y;
-          ay += g.x * xw.y + g.y * xw.x;
+        cf wk = circle(roots, n, ((uint32_t)f0 * k) & mask);
+        const cf step = circle(roots, n, k & mask);
+#pragma unroll
+        for (int q = 0; q < F; q++) {
+          if (q > 0) wk = cmul(wk, step);
+          // templates: conj(g) x w ; data: g x conj(w)
+          cf xw;
+          if (TEMPL) {
+            xw = cmul(x[q], wk);
+            ax[q] += g.x * xw.x + g.y * xw.y;
+            ay[q] += g.x * xw.y - g.y * xw.x;
+          } else {
+            xw = cmul_conj(x[q], wk);
+            ax[q] += g.x * xw.x - g.y * xw.y;
+            ay[q] += g.x * xw.y + g.y * xw.x;
+          }
         }
       }
     }
   }
-  part[w][lane] = cf{ax, ay};
+#pragma unroll
+  for (int q = 0; q < F; q++) part[w][lane * F + q] = cf{ax[q], ay[q]};
   __syncthreads();
-  if (w == 0) {
-    const cf s0 = part[0][lane], s1 = part[1][lane], s2 = part[2][lane], s3 = part[3][lane];
-    out[(long long)row * out_stride + f] = cf{(s0.x + s1.x) + (s2.x + s3.x), (s0.y + s1.y) + (s2.y + s3.y)};
+  for (int c = threadIdx.x; c < 64 * F; c += blockDim.x) {
+    float sx = 0.f, sy = 0.f;
+#pragma unroll
+    for (int v = 0; v < W; v++) {
+      sx += part[v][c].x;
+      sy += part[v][c].y;
+    }
+    out[(long long)row * out_stride + fb + c] = cf{sx, sy};
   }
 }
 
@@ -198,17 +219,22 @@ extern "C" int mfh_peak_grad_async(mfh_ctx *ctx, const void *dev_data_c64, long 
   const cf *roots = mfh::unit_circle(c);
   if (!roots) return -1;
   const int n = (int)c->n;
-  const dim3 block(256);
+  const dim3 block(512);
+  // four frequencies per lane where n allows (n >= 256), else one
+  const bool f4 = n >= 256;
+  const int fcols = n / (f4 ? 256 : 64);
+  auto *ktmpl = f4 ? &mfh::peak_grad_kernel<true, 4> : &mfh::peak_grad_kernel<true, 1>;
+  auto *kdata = f4 ? &mfh::peak_grad_kernel<false, 4> : &mfh::peak_grad_kernel<false, 1>;
   if (dev_grad_tmpl_c64) {
-    hipLaunchKernelGGL(mfh::peak_grad_kernel<true>, dim3(n / 64, nt), block, 0, c->stream,
-                       (const cf *)dev_data_c64, data_stride, nd, dev_idx, (const cf *)dev_gval_c64, nd, nt, (int)nb,
-                       roots, n, (cf *)dev_grad_tmpl_c64, (long long)n);
+    hipLaunchKernelGGL(ktmpl, dim3(fcols, nt), block,
+                       0, c->stream, (const cf *)dev_data_c64, data_stride, nd, dev_idx, (const cf *)dev_gval_c64, nd,
+                       nt, (int)nb, roots, n, (cf *)dev_grad_tmpl_c64, (long long)n);
     if (!hip_ok(hipGetLastError(), "peak_grad_kernel")) return -1;
   }
   if (dev_grad_data_c64) {
-    hipLaunchKernelGGL(mfh::peak_grad_kernel<false>, dim3(n / 64, nd), block, 0, c->stream,
-                       (const cf *)dev_tmpl_c64, tmpl_stride, nt, dev_idx, (const cf *)dev_gval_c64, nd, nt, (int)nb,
-                       roots, n, (cf *)dev_grad_data_c64, (long long)n);
+    hipLaunchKernelGGL(kdata, dim3(fcols, nd), block,
+                       0, c->stream, (const cf *)dev_tmpl_c64, tmpl_stride, nt, dev_idx, (const cf *)dev_gval_c64, nd,
+                       nt, (int)nb, roots, n, (cf *)dev_grad_data_c64, (long long)n);
     if (!hip_ok(hipGetLastError(), "peak_grad_kernel")) return -1;
   }
   return 0;

@@ -357,3 +357,35 @@ def test_repeated_calls_alternate_counters_and_graphs(graphs, gpu_device):
         np.testing.assert_array_equal(out["value"], val, err_msg=ctx)
         assert h.stats[1] - before == refined, ctx
         assert 0 < refined < nd * nt, ctx
+
+
+@pytest.mark.parametrize("prec", ["fp16", "fp32"])
+def test_refine_grid_sized_from_a_stale_count_drains_every_survivor(prec, gpu_device):
+    """The refine launch is sized from the previous call's survivor count: a
+    call that refines every pair right after one that refined none (the
+    smallest grid) must still drain the whole list, and the reverse must
+    give the noise answer; each against the flat filter on the same data."""
+    n, nd, nt = 4096, 16, 1024
+    rng = np.random.default_rng(41)
+    power = inspiral_power(n)
+    H = np.stack([template_with_power(n, power, rng) for _ in range(nt)])
+    D = noise((nd, n), rng)
+    flat, hier = build(n, nd, nt, power, band=512)
+    hier.set_coarse_precision(prec)
+    for f in (flat, hier):
+        f.set_templates(H)
+        f.set_data(D)
+    ref = flat.run(binsize=n, threshold=4.5)
+    for step, gate in enumerate((None, 0.0, None, 0.0)):
+        hier.set_coarse_threshold(gate)
+        before = hier.stats[1]
+        out = hier.run(binsize=n, threshold=4.5)
+        refined = hier.stats[1] - before
+        ctx = "step %d, %s, gate %s" % (step, prec, gate)
+        if gate == 0.0:
+            assert refined == nd * nt, ctx
+            np.testing.assert_array_equal(out["index"], ref["index"], err_msg=ctx)
+            np.testing.assert_array_equal(out["value"], ref["value"], err_msg=ctx)
+        else:
+            assert refined < nd * nt // 8, ctx
+            assert_one_sided(ref, out, ctx)

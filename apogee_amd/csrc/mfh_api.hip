@@ -95,12 +95,21 @@ bool peak_buffer(Ctx *c, size_t cells, PeakBuf &b) {
 long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts) {
   const size_t bi = cells * sizeof(int64_t), bytes = bi + cells * sizeof(cf);
   char *st = (char *)c->host_stage(bytes);
-  if (!st) return -1;
-  if (!hip_ok(hipMemcpyAsync(st, b.idx, bytes, hipMemcpyDeviceToHost, c->stream), "copy results") ||
-      !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
-    return -1;
-  memcpy(idx, st, bi);
-  memcpy(val_c64, st + bi, bytes - bi);
+  if (st) {
+    if (!hip_ok(hipMemcpyAsync(st, b.idx, bytes, hipMemcpyDeviceToHost, c->stream), "copy results") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+      return -1;
+    memcpy(idx, st, bi);
+    memcpy(val_c64, st + bi, bytes - bi);
+  } else {
+    // no page-locked staging to be had: copy straight into the caller's
+    // (pageable) arrays, one copy each
+    (void)hipGetLastError();
+    if (!hip_ok(hipMemcpyAsync(idx, b.idx, bi, hipMemcpyDeviceToHost, c->stream), "copy idx") ||
+        !hip_ok(hipMemcpyAsync(val_c64, b.val, bytes - bi, hipMemcpyDeviceToHost, c->stream), "copy val") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+      return -1;
+  }
   long total = 0;
   for (size_t p = 0; p < cells / nb; p++) {
     int cnt = 0;

@@ -29,7 +29,7 @@ __device__ __forceinline__ cf cmul_conj(cf a, cf b) {
 
 constexpr float kSqrtHalf = 0.70710678118654752440f;
 constexpr float kCos8 = 0.92387953251128675613f;  // cos(pi/8)
-constexpr float kSin8 = 0.38268343236508977173f;  // sin(pi/8)
+constexpr float kTan8 = 0.41421356237309504880f;  // tan(pi/8) = sqrt(2) - 1
 
 // a * exp(+i pi/4)
 __device__ __forceinline__ cf rot_w8_1(cf a) {
@@ -39,12 +39,6 @@ __device__ __forceinline__ cf rot_w8_1(cf a) {
 __device__ __forceinline__ cf rot_w8_3(cf a) {
   return {-(a.x + a.y) * kSqrtHalf, (a.x - a.y) * kSqrtHalf};
 }
-// a * exp(+i pi/8)
-__device__ __forceinline__ cf rot_w16_1(cf a) { return cmul(a, cf{kCos8, kSin8}); }
-// a * exp(+3 i pi/8)
-__device__ __forceinline__ cf rot_w16_3(cf a) { return cmul(a, cf{kSin8, kCos8}); }
-// a * exp(+9 i pi/8) = -(a * exp(+i pi/8))
-__device__ __forceinline__ cf rot_w16_9(cf a) { return cmul(a, cf{-kCos8, -kSin8}); }
 
 __device__ __forceinline__ void dft2(cf &a0, cf &a1) {
   const cf s = cadd(a0, a1), d = csub(a0, a1);
@@ -81,25 +75,6 @@ __device__ __forceinline__ void dft8(cf *v) {
   for (int i = 0; i < 8; i++) v[i] = o[i];
 }
 
-// The second-layer 4-point DFT of a 16-point transform whose third input
-// still owes the factor w8 = exp(+i pi/4) (SIGN = +1) or w8^3 (SIGN = -1):
-// a2 = b2 * w8^(2 - SIGN).  The constant multiply folds into the sum and the
-// difference with a0 as two FMAs per component, where the separate rotation
-// costs a multiply and an add each.
-template <int SIGN>
-__device__ __forceinline__ void dft4_w8_third(cf &a0, cf &a1, cf &b2, cf &a3) {
-  // b2 * w8 = ((x - y) k, (x + y) k); b2 * w8^3 = (-(x + y) k, (x - y) k)
-  const float p = SIGN > 0 ? b2.x - b2.y : -(b2.x + b2.y);
-  const float q = SIGN > 0 ? b2.x + b2.y : b2.x - b2.y;
-  const cf s0 = {__builtin_fmaf(p, kSqrtHalf, a0.x), __builtin_fmaf(q, kSqrtHalf, a0.y)};
-  const cf d0 = {__builtin_fmaf(-p, kSqrtHalf, a0.x), __builtin_fmaf(-q, kSqrtHalf, a0.y)};
-  const cf s1 = cadd(a1, a3), d1 = cmul_i(csub(a1, a3));
-  a0 = cadd(s0, s1);
-  b2 = csub(s0, s1);
-  a1 = cadd(d0, d1);
-  a3 = csub(d0, d1);
-}
-
 // The second-layer 4-point DFT for k1 = 2, inputs owing 1, w8, i, w8^3:
 // with u = b1 + i b3 and v = b1 - i b3, a1 + a3 = w8 u and
 // i (a1 - a3) = w8^3 v, and both rotations fold into the output FMAs.
@@ -116,20 +91,45 @@ __device__ __forceinline__ void dft4_k2(cf &a0, cf &b1, cf &b2, cf &b3) {
   b3 = cf{__builtin_fmaf(qv, kSqrtHalf, d0.x), __builtin_fmaf(-pv, kSqrtHalf, d0.y)};
 }
 
+// The second-layer 4-point DFT for k1 = 1 (SIGN = +1) or k1 = 3 (SIGN =
+// -1), whose inputs owe 1, w16^k1, w8^(2 - SIGN) and w16^(3 k1).  Every odd
+// power of w16 is cos(pi/8) times (1 + i tan(pi/8)) or (tan(pi/8) + i), up to
+// sign: w16 = c (1 + i t), w16^3 = c (t + i), w16^9 = -c (1 + i t) with
+// c = cos(pi/8), t = tan(pi/8).  So b1 and b3 are rotated by the unscaled
+// factors (two FMAs each, against four instructions for a full complex
+// multiply) and the common c folds into the output FMAs, as the w8 factor
+// of b2 does.
+template <int SIGN>
+__device__ __forceinline__ void dft4_w16(cf &a0, cf &b1, cf &b2, cf &b3) {
+  const float p = SIGN > 0 ? b2.x - b2.y : -(b2.x + b2.y);
+  const float q = SIGN > 0 ? b2.x + b2.y : b2.x - b2.y;
+  const cf s0 = {__builtin_fmaf(p, kSqrtHalf, a0.x), __builtin_fmaf(q, kSqrtHalf, a0.y)};
+  const cf d0 = {__builtin_fmaf(-p, kSqrtHalf, a0.x), __builtin_fmaf(-q, kSqrtHalf, a0.y)};
+  // (1 + i t) x = (x.x - t x.y, x.y + t x.x); (t + i) x = (t x.x - x.y, t x.y + x.x)
+  const cf e = SIGN > 0 ? b1 : b3;  // the input rotated by (1 + i t)
+  const cf f = SIGN > 0 ? b3 : b1;  // the input rotated by (t + i)
+  const cf pe = {__builtin_fmaf(-kTan8, e.y, e.x), __builtin_fmaf(kTan8, e.x, e.y)};
+  const cf pf = {__builtin_fmaf(kTan8, f.x, -f.y), __builtin_fmaf(kTan8, f.y, f.x)};
+  // a1 + a3 = c su and i (a1 - a3) = c i dv, where for k1 = 1 a1 = c pe and
+  // a3 = c pf, and for k1 = 3 a1 = c pf and a3 = -c pe
+  const cf su = SIGN > 0 ? cadd(pe, pf) : csub(pf, pe);
+  const cf dv = SIGN > 0 ? csub(pe, pf) : cadd(pf, pe);
+  a0 = cf{__builtin_fmaf(su.x, kCos8, s0.x), __builtin_fmaf(su.y, kCos8, s0.y)};
+  b2 = cf{__builtin_fmaf(-su.x, kCos8, s0.x), __builtin_fmaf(-su.y, kCos8, s0.y)};
+  b1 = cf{__builtin_fmaf(-dv.y, kCos8, d0.x), __builtin_fmaf(dv.x, kCos8, d0.y)};
+  b3 = cf{__builtin_fmaf(dv.y, kCos8, d0.x), __builtin_fmaf(-dv.x, kCos8, d0.y)};
+}
+
 // 16-point inverse DFT as 4 x 4: x[4 n1 + n2].
 __device__ __forceinline__ void dft16(cf *v) {
 #pragma unroll
   for (int n2 = 0; n2 < 4; n2++) dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
-  // Y[n2][k1] at v[4 k1 + n2] owes w16^(n2 k1); the w8 powers (n2 k1 = 2,
-  // 6) and i (n2 k1 = 4) are folded into the second layer's arithmetic
-  v[5] = rot_w16_1(v[5]);
-  v[7] = rot_w16_3(v[7]);
-  v[13] = rot_w16_3(v[13]);
-  v[15] = rot_w16_9(v[15]);
+  // Y[n2][k1] at v[4 k1 + n2] owes w16^(n2 k1); every such factor is folded
+  // into the second layer's arithmetic
   dft4(v[0], v[1], v[2], v[3]);
-  dft4_w8_third<+1>(v[4], v[5], v[6], v[7]);
+  dft4_w16<+1>(v[4], v[5], v[6], v[7]);
   dft4_k2(v[8], v[9], v[10], v[11]);
-  dft4_w8_third<-1>(v[12], v[13], v[14], v[15]);
+  dft4_w16<-1>(v[12], v[13], v[14], v[15]);
   // X[k1 + 4 k2] sits at v[4 k1 + k2]
   cf o[16];
 #pragma unroll

@@ -344,7 +344,8 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
   // fragments, so the workgroup pays one memory latency, not one per loop
   // trip.  The data rows at the A-fragment positions f = 128 q + 16 i + r; a
   // row past nd (the wave's last, when nd is not a multiple of R) is read
-  // from row nd - 1 and zeroed below.
+  // from row nd - 1: it is transformed like any row, and its results are
+  // never gated, counted or written.
   cf x[R][8];
 #pragma unroll
   for (int k = 0; k < R; k++) {
@@ -386,17 +387,19 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
   const int t0 = blockIdx.x * tpw;
   const int t1 = d0 < nd ? min(nt, t0 + tpw) : t0;
 
-  // each row scaled by an exact power of two to components below 1/2
+  // each row scaled by an exact power of two to components below 1/2.  The
+  // exponent comes from the largest |component|, taken on the bit patterns
+  // with the sign cleared (nonnegative floats order like their bits): no
+  // NaN canonicalisation per component.  A NaN component gives the row the
+  // largest exponent; its pairs' statistics are NaN either way.
   int sd[R];
 #pragma unroll
   for (int k = 0; k < R; k++) {
-    float dmax = 0.f;
+    uint32_t mb = 0u;
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      if (d0 + k >= nd) x[k][i] = cf{0.f, 0.f};
-      dmax = fmaxf(dmax, fmaxf(fabsf(x[k][i].x), fabsf(x[k][i].y)));
-    }
-    sd[k] = unit_exponent(wave_max_nonneg(dmax)) + 1;  // components below 1/2
+    for (int i = 0; i < 8; i++)
+      mb = max(mb, max(__float_as_uint(x[k][i].x) & 0x7fffffffu, __float_as_uint(x[k][i].y) & 0x7fffffffu));
+    sd[k] = unit_exponent(__uint_as_float(wave_umax(mb))) + 1;  // components below 1/2
 #pragma unroll
     for (int i = 0; i < 8; i++) x[k][i] = cf{ldexpf(x[k][i].x, -sd[k]), ldexpf(x[k][i].y, -sd[k])};
   }
@@ -424,10 +427,23 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
     const int st = LDSC ? tsexp[t - t0] : sexp[t];
 #pragma unroll
     for (int k = 0; k < R; k++) {
-      const float m2 = ldexpf(__uint_as_float(wave_umax(lm[k])), 2 * (st + sd[k]));
       if (d0 + k >= nd) continue;
-      if (cmax2 && lane == 0) cmax2[(long long)(d0 + k) * nt + t] = m2;
-      if (g.thr2 <= 0.f || m2 >= g.thr2) fired[k] |= 1ull << (t - t0);
+      bool fire;
+      if (cmax2) {
+        // the statistic itself is wanted: the wave maximum, rescaled
+        const float m2 = ldexpf(__uint_as_float(wave_umax(lm[k])), 2 * (st + sd[k]));
+        if (lane == 0) cmax2[(long long)(d0 + k) * nt + t] = m2;
+        fire = g.thr2 <= 0.f || m2 >= g.thr2;
+      } else {
+        // only the decision: ldexp is monotonic, so "the rescaled wave
+        // maximum reaches thr2" is "some lane's rescaled maximum does" --
+        // unless a lane holds a NaN (bits above +inf), which the maximum
+        // would return and no comparison passes
+        const float m2l = ldexpf(__uint_as_float(lm[k]), 2 * (st + sd[k]));
+        const bool any_nan = __ballot(lm[k] > 0x7f800000u) != 0ull;
+        fire = !any_nan && (g.thr2 <= 0.f || __ballot(m2l >= g.thr2) != 0ull);
+      }
+      if (fire) fired[k] |= 1ull << (t - t0);
     }
   };
 
@@ -484,6 +500,14 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
     base += __popcll(f);
     if (__popcll(f) == cnt) continue;
     const long long o = ((long long)(d0 + k) * nt + t0) * g.nb;
+    if (g.nb == 1) {
+      // one bin per pair (tpw <= 64): lane j writes template t0 + j
+      if (lane < cnt && !((f >> lane) & 1ull)) {
+        g.idx[o + lane] = -1;
+        g.val[o + lane] = cf{0.f, 0.f};
+      }
+      continue;
+    }
     for (int e = lane; e < cnt * g.nb; e += 64) {
       const int j = e / g.nb;
       if (!((f >> j) & 1ull)) {

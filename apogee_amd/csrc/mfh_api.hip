@@ -470,6 +470,11 @@ extern "C" int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value) {
     c->coarse_lds = (int)value;
     return 0;
   }
+  if (!strcmp(key, "coarse_fuse")) {
+    if (value != 0 && value != 1) return fail("coarse_fuse must be 0 or 1");
+    c->coarse_fuse = (int)value;
+    return 0;
+  }
   if (!strcmp(key, "refine_grid")) {
     if (value < 1) return fail("refine_grid must be >= 1");
     c->refine_grid_cap = (int)value;

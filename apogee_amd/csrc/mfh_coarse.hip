@@ -25,6 +25,7 @@
 #include <cmath>
 #include <vector>
 
+#include "mfh_flat.h"
 #include "mfh_internal.h"
 #include "mfh_mfma.h"
 #include "mfh_wave.h"
@@ -327,17 +328,19 @@ __device__ __forceinline__ void load_frag8(cf *ct, const cf *ctm, int t, int lan
 // LDSC: the DFT constants are copied once per workgroup into LDS and read
 // where used instead of held in registers (46 fewer VGPRs per lane, so more
 // waves fit per SIMD).
+//
+// This is the screen of one workgroup, shared by the two kernels below: on
+// return fired[k] holds, for data row d0 + k, bit j set iff template t0 + j
+// passed the gate.  LDSC's LDS: ctab (12 x 64 words4, the DFT constants),
+// tsexp (the workgroup's templates' scale exponents, tpw <= 64) and tfrag
+// (the workgroup's templates' fragments, [template][quad][lane] in 16-byte
+// units, conflict-free reads), staged once for the four waves.
 template <bool BF16, int R, bool FULL, bool LDSC>
-__global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long long data_stride, const cf *ctm,
-                                                          const int *sexp, int nd, int nt, int tpw, int cs, int ce,
-                                                          const uint32_t *consts, float *cmax2, CoarseGate g) {
+__device__ __forceinline__ void coarse_screen(const cf *data, long long data_stride, const cf *ctm, const int *sexp,
+                                              int nd, int nt, int tpw, int cs, int ce, const uint32_t *consts,
+                                              float *cmax2, const CoarseGate g, uint4 *ctab, int *tsexp,
+                                              uint4 *tfrag, int d0, int t0, int t1, uint64_t *fired) {
   const int lane = threadIdx.x & 63;
-  __shared__ uint4 ctab[LDSC ? 12 * 64 : 1];
-  __shared__ int tsexp[LDSC ? 64 : 1];  // LDSC: the workgroup's templates' scale exponents (tpw <= 64)
-  // LDSC: the workgroup's templates' fragments, [template][quad][lane] in
-  // 16-byte units (conflict-free reads), staged once for its four waves
-  extern __shared__ uint4 tfrag[];
-  const int d0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * R;
   const int q = lane >> 4, r = lane & 15;
   // Every global load of the prologue is issued before anything waits on
   // one: the data rows, then (LDSC) the constants and the template
@@ -382,10 +385,6 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
     __syncthreads();
   }
   if (g.zero && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *g.zero = 0;
-  // a wave past the last data row walks no templates but stays for the
-  // workgroup's barriers at the end
-  const int t0 = blockIdx.x * tpw;
-  const int t1 = d0 < nd ? min(nt, t0 + tpw) : t0;
 
   // each row scaled by an exact power of two to components below 1/2.  The
   // exponent comes from the largest |component|, taken on the bit patterns
@@ -410,7 +409,6 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
   // chains the scheduler interleaves), then the reductions; gate decisions
   // are kept as wave-uniform bits per row and template (tpw <= 64) and
   // written out once, after the last template
-  uint64_t fired[R];
 #pragma unroll
   for (int k = 0; k < R; k++) fired[k] = 0;
   auto pairs = [&](const cf *ct, int t) {
@@ -471,6 +469,32 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
       pairs(cb, t + 1);
     }
   }
+}
+
+// A wave's first data row d0 and its templates [t0, t1): the workgroup's
+// four waves take 4 R consecutive rows and the same tpw templates.  A wave
+// past the last data row walks no templates but stays for the workgroup's
+// barriers.
+template <int R>
+__device__ __forceinline__ void coarse_tile(int nd, int nt, int tpw, int &d0, int &t0, int &t1) {
+  d0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * R;
+  t0 = blockIdx.x * tpw;
+  t1 = d0 < nd ? min(nt, t0 + tpw) : t0;
+}
+
+template <bool BF16, int R, bool FULL, bool LDSC>
+__global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long long data_stride, const cf *ctm,
+                                                          const int *sexp, int nd, int nt, int tpw, int cs, int ce,
+                                                          const uint32_t *consts, float *cmax2, CoarseGate g) {
+  __shared__ uint4 ctab[LDSC ? 12 * 64 : 1];
+  __shared__ int tsexp[LDSC ? 64 : 1];
+  extern __shared__ uint4 tfrag[];
+  const int lane = threadIdx.x & 63;
+  int d0, t0, t1;
+  coarse_tile<R>(nd, nt, tpw, d0, t0, t1);
+  uint64_t fired[R];
+  coarse_screen<BF16, R, FULL, LDSC>(data, data_stride, ctm, sexp, nd, nt, tpw, cs, ce, consts, cmax2, g, ctab,
+                                     tsexp, tfrag, d0, t0, t1, fired);
   if (!g.list) return;  // uniform over the grid
 
   // survivors: one atomic per workgroup reserves their list slots (a single
@@ -515,6 +539,92 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
         g.val[o + e] = cf{0.f, 0.f};
       }
     }
+  }
+}
+
+// Screen and refine in one launch (transform length FN, one bin, the
+// constants in LDS): each workgroup gates its pairs exactly as
+// coarse_mfma_kernel does, then runs the flat filter's per-pair path on its
+// own survivors with all 256 threads (one FN = 4096 transform at a time), so
+// no survivor list leaves the workgroup and no second launch waits on the
+// first.  The number of survivors still reaches g.count (one atomic per
+// workgroup that has any) for the caller's refinement count.  The screen's
+// LDS (constants, exponents, fragments) and the refine's (exchange buffer,
+// bin keys) share one dynamic allocation; the refine starts after a barrier
+// that ends every read of the screen's.  A workgroup's refines run one after
+// another, so a workgroup with many survivors finishes late: the split is
+// for screens that pass few pairs.
+template <bool BF16, int R, bool FULL, int FN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+coarse_refine_kernel(const cf *data, long long data_stride, const cf *ctm, const int *sexp, int nd, int nt, int tpw,
+                     int cs, int ce, const uint32_t *consts, CoarseGate g, FlatArgs fa) {
+  using S = FlatShape<FN>;
+  static_assert(S::WG == 256 && S::SLOTS == 1 && S::TPT == 256, "one transform per 256-thread workgroup");
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint4 *ctab = reinterpret_cast<uint4 *>(smem);
+  int *tsexp = reinterpret_cast<int *>(ctab + 12 * 64);
+  uint4 *tfrag = reinterpret_cast<uint4 *>(tsexp + 64);
+  __shared__ int2 surv[4 * R * 16];  // the workgroup's survivors (tpw <= 16)
+  __shared__ int wave_cnt[4], wg_n;
+  const int lane = threadIdx.x & 63;
+  int d0, t0, t1;
+  coarse_tile<R>(nd, nt, tpw, d0, t0, t1);
+  uint64_t fired[R];
+  coarse_screen<BF16, R, FULL, true>(data, data_stride, ctm, sexp, nd, nt, tpw, cs, ce, consts, nullptr, g, ctab,
+                                     tsexp, tfrag, d0, t0, t1, fired);
+
+  // survivors into the workgroup's list; the other pairs get -1 / 0
+  const int w = threadIdx.x >> 6;
+  int total = 0;
+#pragma unroll
+  for (int k = 0; k < R; k++) total += __popcll(fired[k]);
+  if (lane == 0) wave_cnt[w] = total;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int sum = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
+    wg_n = sum;
+    if (sum) atomicAdd(g.count, sum);
+  }
+  int base = 0;
+  for (int i = 0; i < w; i++) base += wave_cnt[i];
+  const int cnt = t1 - t0;
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    if (d0 + k >= nd) break;
+    const uint64_t f = fired[k];
+    const bool mine = lane < cnt && ((f >> lane) & 1ull);
+    if (mine) surv[base + __popcll(f & ((1ull << lane) - 1ull))] = int2{d0 + k, t0 + lane};
+    base += __popcll(f);
+    // one bin per pair: lane j answers template t0 + j
+    const long long o = (long long)(d0 + k) * nt + t0 + lane;
+    if (lane < cnt && !mine) {
+      g.idx[o] = -1;
+      g.val[o] = cf{0.f, 0.f};
+    }
+  }
+  __syncthreads();  // the list is complete and the screen's LDS no longer read
+  const int ns = wg_n;
+  if (ns == 0) return;  // workgroup-uniform
+
+  // the flat kernel's per-pair path (list mode), pair after pair
+  cf *xch = reinterpret_cast<cf *>(smem);
+  uint64_t *bins_all = reinterpret_cast<uint64_t *>(xch + S::XSTRIDE);
+  cf wfirst[15];
+  first_twiddles<FN>(wfirst, (int)threadIdx.x, fa.twiddle);
+  if (threadIdx.x < 2) bins_all[threadIdx.x] = 0ull;
+  __syncthreads();
+  for (int i = 0; i < ns; i++) {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const int2 pr = surv[i];
+    cf dv[16], tv[16], v[16];
+    load_row_uniform<FN>(dv, uniform_ptr(fa.data + (long long)pr.x * fa.data_stride), t);
+    load_row_uniform<FN>(tv, uniform_ptr(fa.tmpl + (long long)pr.y * fa.tmpl_stride), t);
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = cmul_conj(dv[u], tv[u]);
+    inverse_fft<FN, true>(v, t, xch, fa.twiddle, wfirst);
+    select_and_write<FN, kOneBin, false>(fa, 1, v, t, lane, bins_all + (i & 1), bins_all + ((i + 1) & 1), true, pr.x,
+                                         pr.y, 0, 0);
   }
 }
 
@@ -607,6 +717,48 @@ int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, int t0, in
   hipLaunchKernelGGL(kern, grid, dim3(256), smem, c->stream, data, data_stride, ctm, sexp, nd, nt, tpw, cs, ce,
                      consts, cmax2, g);
   return hip_ok(hipGetLastError(), "coarse_mfma_kernel") ? 0 : -1;
+}
+
+bool coarse_refine_fits(const Ctx *c, size_t n, size_t nb) {
+  return c->coarse_fuse && n == 4096 && nb == 1 && c->coarse_band == kBand && c->coarse_lds && c->coarse_tpw >= 1 &&
+         c->coarse_tpw <= 16;
+}
+
+int launch_coarse_refine(Ctx *c, const cf *data, long long data_stride, int t0, int nd, int nt, int cs, int ce,
+                         bool bf16, const CoarseGate &g, const FlatRun &rr) {
+  constexpr int FN = 4096;
+  if (c->coarse_band != kBand || !c->coarse_ctm) return fail("the MFMA coarse screen is built for band 512");
+  if (!coarse_refine_fits(c, FN, 1)) return fail("the one-launch screen and refine does not apply here");
+  const uint32_t *consts = coarse_consts(c, bf16);
+  if (!consts) return -1;
+  FlatArgs fa;
+  if (flat_fill_args(c, FN, rr, fa) < 0) return -1;
+  if (fa.nb != 1) return fail("the one-launch screen and refine writes one bin per pair");
+  const int tpw = c->coarse_tpw;  // <= 16: the survivor list holds 4 R tpw pairs
+  const int rows = c->coarse_rpw == 1 ? 1 : 2;
+  dim3 grid((unsigned)((nt + tpw - 1) / tpw), (unsigned)((nd + 4 * rows - 1) / (4 * rows)));
+  if (grid.y > 65535) return fail("too many data rows per launch (%d)", nd);
+  const cf *ctm = c->coarse_ctm + (size_t)t0 * kBand;
+  const int *sexp = c->coarse_sexp + t0;
+  const bool full = cs == 0 && ce == kBand;
+  using Kern = decltype(&coarse_refine_kernel<true, 1, true, FN>);
+  // [bf16][rows - 1][full]
+  static const Kern table[2][2][2] = {
+      {{coarse_refine_kernel<false, 1, false, FN>, coarse_refine_kernel<false, 1, true, FN>},
+       {coarse_refine_kernel<false, 2, false, FN>, coarse_refine_kernel<false, 2, true, FN>}},
+      {{coarse_refine_kernel<true, 1, false, FN>, coarse_refine_kernel<true, 1, true, FN>},
+       {coarse_refine_kernel<true, 2, false, FN>, coarse_refine_kernel<true, 2, true, FN>}}};
+  const Kern kern = table[bf16 ? 1 : 0][rows - 1][full ? 1 : 0];
+  // the screen's constants, exponents and fragments, or the refine's
+  // exchange buffer and two one-bin key slots, whichever is larger
+  const size_t screen = 12 * 64 * sizeof(uint4) + 64 * sizeof(int) + (size_t)tpw * 4096;
+  const size_t refine = (size_t)FlatShape<FN>::XSTRIDE * sizeof(cf) + 2 * sizeof(uint64_t);
+  const size_t smem = std::max(screen, refine);
+  if (!lds_limit((const void *)kern, c->dev, smem)) return -1;
+  Ctx::ProfScope ps(c, "coarse");
+  hipLaunchKernelGGL(kern, grid, dim3(256), smem, c->stream, data, data_stride, ctm, sexp, nd, nt, tpw, cs, ce,
+                     consts, g, fa);
+  return hip_ok(hipGetLastError(), "coarse_refine_kernel") ? 0 : -1;
 }
 
 }  // namespace mfh

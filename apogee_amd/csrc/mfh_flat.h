@@ -426,6 +426,167 @@ enum BinMode : int {
   kGeneral = 2,  // anything else
 };
 
+// The end of one pair's work in the flat kernel, after its inverse FFT has
+// left z[t + u*TPT] in v: the per-bin keys of the lags in the window, the
+// workgroup's bin maxima in LDS, and the writes of each bin's winning lag and
+// value (-1 / 0 for bins nothing reached).  Every thread of the workgroup
+// calls it (it holds a barrier); `valid` says whether this slot's pair is
+// real, (d, tt) are its data row and template, nb = a.nb, and its output row starts at
+// list_ooff[dd + slot] (list mode with offsets) or (d nt + tt) ostride.
+// Clears `bins_next` for the next pair.
+template <int N, int MODE, bool LIST>
+__device__ __forceinline__ void select_and_write(const FlatArgs &a, int nb, const cf *v, int t, int lane,
+                                                 uint64_t *bins, uint64_t *bins_next, bool valid, int d, int tt,
+                                                 int dd, int slot) {
+  constexpr int TPT = FlatShape<N>::TPT;
+  if constexpr (MODE == kOneBin) {
+    // ---- one bin: a wave key, one LDS max; the winning lag's owner writes ----
+    uint64_t wkey = 0ull;
+    bool have = false;
+    if constexpr (TPT >= 64) {
+      if (a.ws == 0 && a.we == (uint32_t)N) have = one_bin_wave_key<TPT>(v, t, a.seed, wkey);
+    }
+    if (!have) {
+      // per-sample scan: strict, so the lowest lag keeps a tie
+      float bm = a.seed;
+      int bu = -1;
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const uint32_t k = (uint32_t)(t + u * TPT);
+        const float m2 = __builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y);
+        if (k >= a.ws && k < a.we && m2 > bm) {
+          bm = m2;
+          bu = u;
+        }
+      }
+      const uint64_t key =
+          bu >= 0 ? ((uint64_t)__float_as_uint(bm) << 32) | (uint64_t)(~(uint32_t)(t + bu * TPT)) : 0ull;
+      if constexpr (TPT >= 64) wkey = wave_max_u64(key);
+      else wkey = key;  // several slots per wave: every lane raises its own key
+    }
+    if (TPT >= 64 ? (lane == 0 && wkey) : wkey != 0ull)
+      atomicMax((unsigned long long *)&bins[0], (unsigned long long)wkey);
+    __syncthreads();
+    if (valid) {
+      const long long o = (LIST && a.list_ooff) ? a.list_ooff[dd + slot] : ((long long)d * a.nt + tt) * a.ostride;
+      const uint64_t win = bins[0];
+      const uint32_t kwin = ~(uint32_t)win;
+      if (win && (int)(kwin % TPT) == t) {
+        // lag kwin = t + uw * TPT is this thread's sample uw
+        const int uw = (int)(kwin / TPT);
+        // a select per sample on the bit patterns, so that v stays in
+        // registers (an indexed read would put the array in scratch)
+        uint32_t zx = 0u, zy = 0u;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          zx |= u == uw ? __float_as_uint(v[u].x) : 0u;
+          zy |= u == uw ? __float_as_uint(v[u].y) : 0u;
+        }
+        const cf z = {__uint_as_float(zx), __uint_as_float(zy)};
+        a.out_idx[o] = (int64_t)kwin;
+        a.out_val[o] = z;
+      }
+      if (t == 0 && win == 0ull) {
+        a.out_idx[o] = -1;
+        a.out_val[o] = cf{0.f, 0.f};
+      }
+    }
+  } else {
+    // ---- several bins: per-thread runs, wave reductions, LDS max ----------
+    // whole-transform window with power-of-two bins of >= 64 lags: the
+    // transposing butterfly (wave-uniform test)
+    const bool full = MODE == kAligned && a.ws == 0 && a.we == (uint32_t)N && a.bshift >= 6;
+    if constexpr (TPT >= 64) {
+      if (full && full_window_bins<TPT>(v, t, lane, a.seed, a.bshift, bins)) {
+      } else {
+        uint64_t acc = 0ull;
+        int acc_bin = -1;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const uint32_t k = (uint32_t)(t + u * TPT);
+          const uint64_t key = (k >= a.ws && k < a.we) ? make_key(v[u], k, a.seed) : 0ull;
+          bool uni;
+          uint32_t kf = 0;
+          if constexpr (MODE == kAligned) {
+            kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
+            uni = kf >= a.ws && kf < a.we;
+          } else {
+            kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
+            uni = kf >= a.ws && kf + 63 < a.we && bin_of(kf, a) == bin_of(kf + 63, a);
+          }
+          if (uni) {
+            const int b = (int)bin_of(kf, a);
+            if (b != acc_bin) {
+              if (acc_bin >= 0) {
+                const uint64_t w = wave_max_u64(acc);
+                if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
+              }
+              acc = 0ull;
+              acc_bin = b;
+            }
+            acc = key > acc ? key : acc;
+          } else if (key) {
+            atomicMax((unsigned long long *)&bins[bin_of(k, a)], (unsigned long long)key);
+          }
+        }
+        if (acc_bin >= 0) {
+          const uint64_t w = wave_max_u64(acc);
+          if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
+        }
+      }
+    } else {
+      // several transforms share a wave: per-thread runs, then LDS atomics
+      uint64_t acc = 0ull;
+      int acc_bin = -1;
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const uint32_t k = (uint32_t)(t + u * TPT);
+        if (k >= a.ws && k < a.we) {
+          const uint64_t key = make_key(v[u], k, a.seed);
+          const int b = (int)bin_of(k, a);
+          if (b != acc_bin) {
+            if (acc) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)acc);
+            acc = 0ull;
+            acc_bin = b;
+          }
+          acc = key > acc ? key : acc;
+        }
+      }
+      if (acc) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)acc);
+    }
+    __syncthreads();
+
+    // ---- the owner of each winning lag writes it; empty bins get -1/0 ----
+    if (valid) {
+      const long long o = (LIST && a.list_ooff) ? a.list_ooff[dd + slot] : ((long long)d * a.nt + tt) * a.ostride;
+      int64_t *oidx = a.out_idx + o;
+      cf *oval = a.out_val + o;
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const uint32_t k = (uint32_t)(t + u * TPT);
+        if (k >= a.ws && k < a.we) {
+          const uint64_t key = make_key(v[u], k, a.seed);
+          if (key) {
+            const uint32_t b = bin_of(k, a);
+            if (bins[b] == key) {
+              oidx[b] = (int64_t)k;
+              oval[b] = v[u];
+            }
+          }
+        }
+      }
+      for (int b = t; b < nb; b += TPT) {
+        if (bins[b] == 0ull) {
+          oidx[b] = -1;
+          oval[b] = cf{0.f, 0.f};
+        }
+      }
+    }
+  }
+  // the other buffer was last read before this pair's first barrier
+  for (int b = t; b < nb; b += TPT) bins_next[b] = 0ull;
+}
+
 template <int N, int MODE, bool LIST>
 __global__ void __launch_bounds__(FlatShape<N>::WG) __attribute__((amdgpu_waves_per_eu(4)))
 flat_kernel(FlatArgs a) {
@@ -521,148 +682,7 @@ flat_kernel(FlatArgs a) {
       for (int u = 0; u < 16; u++) v[u] = cmul_conj(dv[u], tv[u]);
     }
     inverse_fft<N, true>(v, t, xch, a.twiddle, wfirst);
-
-    if constexpr (MODE == kOneBin) {
-      // ---- one bin: a wave key, one LDS max; the winning lag's owner writes ----
-      uint64_t wkey = 0ull;
-      bool have = false;
-      if constexpr (TPT >= 64) {
-        if (a.ws == 0 && a.we == (uint32_t)N) have = one_bin_wave_key<TPT>(v, t, a.seed, wkey);
-      }
-      if (!have) {
-        // per-sample scan: strict, so the lowest lag keeps a tie
-        float bm = a.seed;
-        int bu = -1;
-#pragma unroll
-        for (int u = 0; u < 16; u++) {
-          const uint32_t k = (uint32_t)(t + u * TPT);
-          const float m2 = __builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y);
-          if (k >= a.ws && k < a.we && m2 > bm) {
-            bm = m2;
-            bu = u;
-          }
-        }
-        const uint64_t key =
-            bu >= 0 ? ((uint64_t)__float_as_uint(bm) << 32) | (uint64_t)(~(uint32_t)(t + bu * TPT)) : 0ull;
-        if constexpr (TPT >= 64) wkey = wave_max_u64(key);
-        else wkey = key;  // several slots per wave: every lane raises its own key
-      }
-      if (TPT >= 64 ? (lane == 0 && wkey) : wkey != 0ull)
-        atomicMax((unsigned long long *)&bins[0], (unsigned long long)wkey);
-      __syncthreads();
-      if (valid) {
-        const long long o = (LIST && a.list_ooff) ? a.list_ooff[dd + slot] : ((long long)d * a.nt + tt) * a.ostride;
-        const uint64_t win = bins[0];
-        const uint32_t kwin = ~(uint32_t)win;
-        if (win && (int)(kwin % TPT) == t) {
-          // lag kwin = t + uw * TPT is this thread's sample uw
-          const int uw = (int)(kwin / TPT);
-          cf z = v[0];
-#pragma unroll
-          for (int u = 1; u < 16; u++)
-            if (u == uw) z = v[u];
-          a.out_idx[o] = (int64_t)kwin;
-          a.out_val[o] = z;
-        }
-        if (t == 0 && win == 0ull) {
-          a.out_idx[o] = -1;
-          a.out_val[o] = cf{0.f, 0.f};
-        }
-      }
-    } else {
-      // ---- several bins: per-thread runs, wave reductions, LDS max ----------
-      // whole-transform window with power-of-two bins of >= 64 lags: the
-      // transposing butterfly (wave-uniform test)
-      const bool full = MODE == kAligned && a.ws == 0 && a.we == (uint32_t)N && a.bshift >= 6;
-      if constexpr (TPT >= 64) {
-        if (full && full_window_bins<TPT>(v, t, lane, a.seed, a.bshift, bins)) {
-        } else {
-          uint64_t acc = 0ull;
-          int acc_bin = -1;
-#pragma unroll
-          for (int u = 0; u < 16; u++) {
-            const uint32_t k = (uint32_t)(t + u * TPT);
-            const uint64_t key = (k >= a.ws && k < a.we) ? make_key(v[u], k, a.seed) : 0ull;
-            bool uni;
-            uint32_t kf = 0;
-            if constexpr (MODE == kAligned) {
-              kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
-              uni = kf >= a.ws && kf < a.we;
-            } else {
-              kf = (uint32_t)__builtin_amdgcn_readfirstlane((int)(k - lane));
-              uni = kf >= a.ws && kf + 63 < a.we && bin_of(kf, a) == bin_of(kf + 63, a);
-            }
-            if (uni) {
-              const int b = (int)bin_of(kf, a);
-              if (b != acc_bin) {
-                if (acc_bin >= 0) {
-                  const uint64_t w = wave_max_u64(acc);
-                  if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
-                }
-                acc = 0ull;
-                acc_bin = b;
-              }
-              acc = key > acc ? key : acc;
-            } else if (key) {
-              atomicMax((unsigned long long *)&bins[bin_of(k, a)], (unsigned long long)key);
-            }
-          }
-          if (acc_bin >= 0) {
-            const uint64_t w = wave_max_u64(acc);
-            if (lane == 0 && w) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)w);
-          }
-        }
-      } else {
-        // several transforms share a wave: per-thread runs, then LDS atomics
-        uint64_t acc = 0ull;
-        int acc_bin = -1;
-#pragma unroll
-        for (int u = 0; u < 16; u++) {
-          const uint32_t k = (uint32_t)(t + u * TPT);
-          if (k >= a.ws && k < a.we) {
-            const uint64_t key = make_key(v[u], k, a.seed);
-            const int b = (int)bin_of(k, a);
-            if (b != acc_bin) {
-              if (acc) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)acc);
-              acc = 0ull;
-              acc_bin = b;
-            }
-            acc = key > acc ? key : acc;
-          }
-        }
-        if (acc) atomicMax((unsigned long long *)&bins[acc_bin], (unsigned long long)acc);
-      }
-      __syncthreads();
-
-      // ---- the owner of each winning lag writes it; empty bins get -1/0 ----
-      if (valid) {
-        const long long o = (LIST && a.list_ooff) ? a.list_ooff[dd + slot] : ((long long)d * a.nt + tt) * a.ostride;
-        int64_t *oidx = a.out_idx + o;
-        cf *oval = a.out_val + o;
-#pragma unroll
-        for (int u = 0; u < 16; u++) {
-          const uint32_t k = (uint32_t)(t + u * TPT);
-          if (k >= a.ws && k < a.we) {
-            const uint64_t key = make_key(v[u], k, a.seed);
-            if (key) {
-              const uint32_t b = bin_of(k, a);
-              if (bins[b] == key) {
-                oidx[b] = (int64_t)k;
-                oval[b] = v[u];
-              }
-            }
-          }
-        }
-        for (int b = t; b < nb; b += TPT) {
-          if (bins[b] == 0ull) {
-            oidx[b] = -1;
-            oval[b] = cf{0.f, 0.f};
-          }
-        }
-      }
-    }
-    // the other buffer was last read before this pair's first barrier
-    for (int b = t; b < nb; b += TPT) bins_next[b] = 0ull;
+    select_and_write<N, MODE, LIST>(a, nb, v, t, lane, bins, bins_next, valid, d, tt, dd, slot);
   }
 }
 

@@ -37,10 +37,10 @@ int launch_mode(Ctx *c, const FlatArgs &a, dim3 grid, size_t lds) {
   return hip_ok(hipGetLastError(), "flat_kernel launch") ? 0 : -1;
 }
 
+// The kernel arguments of a launch over bins [bin0, bin0 + nb) of r (the
+// lag window [ws, we) of those bins; nb_total bins per output row).
 template <int N>
-int launch_n(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, size_t bin0, size_t nb) {
-  using S = FlatShape<N>;
-  FlatArgs a;
+int fill_args(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, size_t bin0, size_t nb, FlatArgs &a) {
   a.data = r.data;
   a.tmpl = r.tmpl;
   a.twiddle = c->twiddles(N);
@@ -68,10 +68,19 @@ int launch_n(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, si
   a.list_ooff = r.list_ooff;
   a.zero = bin0 == 0 ? r.zero : nullptr;
   a.hint = bin0 == 0 ? r.hint : nullptr;
-  const bool list = r.list != nullptr;
   int dpw = c->tune_dpw;
   if (dpw <= 0) dpw = std::min(r.nd, 4);
   a.dpw = dpw;
+  return 0;
+}
+
+template <int N>
+int launch_n(Ctx *c, const FlatRun &r, size_t ws, size_t we, size_t nb_total, size_t bin0, size_t nb) {
+  using S = FlatShape<N>;
+  FlatArgs a;
+  if (fill_args<N>(c, r, ws, we, nb_total, bin0, nb, a) < 0) return -1;
+  const bool list = r.list != nullptr;
+  const int dpw = a.dpw;
   dim3 grid;
   if (list) {
     // enough workgroups to fill the device; each drains the list grid-stride.
@@ -176,6 +185,16 @@ int flat_fill_twiddles(Ctx *c, size_t n, cf *tw) {
     case 16384: return fill_twiddles<16384>(c, tw);
     default: return fail("transform length %zu is not supported", n);
   }
+}
+
+int flat_fill_args(Ctx *c, size_t n, const FlatRun &r, FlatArgs &a) {
+  if (n != 4096) return fail("flat arguments are built here for n = 4096 only (got %zu)", n);
+  if (r.binsize < 1) return fail("binsize must be >= 1");
+  const size_t we = std::min(r.we, n);
+  if (r.ws >= we) return fail("empty window (%zu, %zu)", r.ws, r.we);
+  const size_t nb = (we - r.ws + r.binsize - 1) / r.binsize;
+  if (nb > flat_max_bins<4096>()) return fail("too many bins for one launch (%zu)", nb);
+  return fill_args<4096>(c, r, r.ws, we, nb, 0, nb, a);
 }
 
 bool flat_supported(size_t n) {

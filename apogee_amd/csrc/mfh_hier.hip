@@ -199,6 +199,31 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
     g.list = list;
     g.count = count;
     g.zero = other;
+    if (coarse_refine_fits(c, n, nb)) {
+      // one launch: each workgroup of the screen refines its own survivors
+      // (the count still goes to this call's counter; no list, no second
+      // launch)
+      FlatRun rr;
+      rr.data = drows;
+      rr.tmpl = c->tmpl + (size_t)t0 * n;
+      rr.data_stride = (long long)n;
+      rr.tmpl_stride = (long long)n;
+      rr.nd = nd;
+      rr.nt = nt;
+      rr.binsize = binsize;
+      rr.threshold = threshold;
+      rr.ws = ws;
+      rr.we = we;
+      rr.out_idx = idx;
+      rr.out_val = val;
+      if (launch_coarse_refine(c, drows, (long long)n, t0, nd, nt, (int)cs, (int)ce, c->coarse_prec == 2, g, rr) < 0)
+        return -1;
+      if (!c->capturing && !hs) c->hier_parity ^= 1;
+      if (count_out &&
+          !hip_ok(hipMemcpyAsync(count_out, count, sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy count"))
+        return -1;
+      return 0;
+    }
     if (launch_coarse_mfma(c, drows, (long long)n, t0, nd, nt, (int)cs, (int)ce, c->coarse_prec == 2, nullptr, &g) <
         0)
       return -1;
@@ -392,6 +417,7 @@ int hier_run_cached(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, floa
                                 fbits(coarse_thr), (long long)(intptr_t)idx, (long long)(intptr_t)val,
                                 c->coarse_band, dbits(c->coarse_ref_f), c->coarse_prec, fbits(c->coarse_delta),
                                 c->refine_grid_cap, (long long)(intptr_t)c->coarse_tmpl, c->coarse_tpw, c->coarse_rpw, c->coarse_lds,
+                                c->coarse_fuse,
                                 (long long)(intptr_t)c->data, (long long)(intptr_t)c->tmpl};
   const int par = c->hier_parity;
   if (key == c->graph_key && c->hier_graph[par]) {

@@ -52,6 +52,7 @@ struct Ctx {
   int coarse_tpw = 4;  // templates per wave of the MFMA coarse screen
   int coarse_rpw = 2;  // data rows per wave of the MFMA coarse screen (1 or 2)
   int coarse_lds = 0;  // 1: the screen reads its DFT constants from LDS, not registers
+  int coarse_fuse = 1;  // 1: where it applies, the matrix-core screen refines its survivors in the same launch
   // the coarse templates in MFMA fragment order, power-of-two scaled, and
   // their scale exponents (built with coarse_tmpl at band 512)
   cf *coarse_ctm = nullptr;
@@ -174,6 +175,10 @@ int flat_fill_twiddles(Ctx *c, size_t n, cf *tw);
 // Validates, splits windows whose bin table would not fit LDS, and launches
 // the length-n flat kernel on c->stream.  Returns 0 or -1 (message set).
 int launch_flat(Ctx *c, size_t n, const FlatRun &r);
+// The flat kernel's arguments for r in one launch (n = 4096, all bins of the
+// window in one bin table), for kernels that run its per-pair path.
+struct FlatArgs;
+int flat_fill_args(Ctx *c, size_t n, const FlatRun &r, FlatArgs &a);
 // The two-kernel four-step flat filter for n = 32768 and 65536.
 int launch_big(Ctx *c, size_t n, const FlatRun &r);
 // Four-step root tables for n = n1 * 256: w_n1^h (h < n1) then w_n^l (l < 256),
@@ -201,6 +206,14 @@ struct CoarseGate {
 };
 int launch_coarse_mfma(Ctx *c, const cf *data, long long data_stride, int t0, int nd, int nt, int cs, int ce,
                        bool bf16, float *cmax2, const CoarseGate *gate = nullptr);
+// Whether a hierarchical run of length n with nb bins takes the one-launch
+// screen-and-refine (matrix-core screen with its constants in LDS, n = 4096,
+// one bin, tuning key coarse_fuse), and that launch: the screen of
+// launch_coarse_mfma with gate g, whose survivors are refined per rr in the
+// same workgroups (g.list is not used).
+bool coarse_refine_fits(const Ctx *c, size_t n, size_t nb);
+int launch_coarse_refine(Ctx *c, const cf *data, long long data_stride, int t0, int nd, int nt, int cs, int ce,
+                         bool bf16, const CoarseGate &g, const FlatRun &rr);
 // Hierarchical run over data rows [d0, d0+nd) of the context, or over
 // `data_rows` (stride n) when not null.
 int *hier_counter(Ctx *c, int parity);

@@ -291,7 +291,10 @@ long mfh_hier_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, i
 /* Tuning knobs: key "dpw" (data rows per workgroup; 0 = automatic),
  * "refine_grid" (workgroups of a list-driven refine), "coarse_tpw"
  * (templates per wave of the matrix-core coarse screen, 1..64), "coarse_rpw"
- * (its data rows per wave, 1 or 2), "big_chunk_mb" (scratch
+ * (its data rows per wave, 1 or 2), "coarse_lds" (0/1: its DFT constants
+ * read from LDS), "coarse_fuse" (0/1, on by default: at n = 4096 with one
+ * bin and the constants in LDS, the screen's workgroups refine their own
+ * survivors in the same launch), "big_chunk_mb" (scratch
  * per four-step launch pair at n = 32768 / 65536), "graphs" (0/1: replay the
  * hierarchical sequence from HIP graphs; off by default, measured slower).
  * Replaces: MF_MFTILE / MF_PBMAX environment knobs (src/matchfilt.c:93-107). */

@@ -11,11 +11,13 @@ Tunables (each a key of the library's tuning interface):
   intermediate no longer stays in the memory-side cache between the two;
 * ``hier`` -- ``refine_grid``: workgroups of the list-driven refine launch,
   which drains however many pairs the coarse gate passed;
-* ``coarse`` -- ``coarse_lds`` x ``coarse_rpw`` x ``coarse_tpw``: where the
-  matrix-core coarse screen keeps its DFT constants (registers, or read from
-  a per-workgroup LDS copy: fewer VGPRs, more waves per SIMD), and the data
-  rows and templates each wave takes (more of either: the per-wave set-up
-  is shared by more pairs; fewer: more waves in flight).  Timed as whole
+* ``coarse`` -- ``coarse_lds`` x ``coarse_rpw`` x ``coarse_tpw`` x
+  ``coarse_fuse``: where the matrix-core coarse screen keeps its DFT
+  constants (registers, or read from a per-workgroup LDS copy: fewer VGPRs,
+  more waves per SIMD), the data rows and templates each wave takes (more of
+  either: the per-wave set-up is shared by more pairs; fewer: more waves in
+  flight), and whether (n = 4096, one bin, constants in LDS) the screen's
+  workgroups refine their own survivors in the same launch.  Timed as whole
   hierarchical calls back to back (wall clock per call, fp16 screen), at a
   gate no noise pair passes and at one about 1% pass: single launches
   bracketed by events rank these splits differently from a stream of calls;
@@ -47,7 +49,7 @@ CANDIDATES = {
     "flat": {"dpw": [1, 2, 4, 8, 16]},
     "big": {"big_chunk_mb": [64, 96, 128, 160, 192, 256]},
     "hier": {"refine_grid": [256, 512, 1024, 2048, 4096]},
-    "coarse": {"coarse_lds": [0, 1], "coarse_rpw": [1, 2], "coarse_tpw": [1, 2, 4, 8, 16]},
+    "coarse": {"coarse_lds": [0, 1], "coarse_rpw": [1, 2], "coarse_tpw": [1, 2, 4, 8, 16], "coarse_fuse": [0, 1]},
     "queue": {"grid_per_cu": [1, 2]},
 }
 
@@ -227,10 +229,14 @@ def _tune_coarse(device, n, nd, nt, band, reps, rng, calls=200):
     gates = [float(cmax[-1]) * 1.01, float(cmax[int(0.99 * cmax.size)])]
     ms = {}
     c = CANDIDATES["coarse"]
-    for lds, rpw, tpw in itertools.product(c["coarse_lds"], c["coarse_rpw"], c["coarse_tpw"]):
+    for lds, rpw, tpw, fuse in itertools.product(c["coarse_lds"], c["coarse_rpw"], c["coarse_tpw"],
+                                                 c["coarse_fuse"]):
+        if fuse and not lds:
+            continue   # the one-launch form reads its constants from LDS
         ctx.set_tuning("coarse_lds", lds)
         ctx.set_tuning("coarse_rpw", rpw)
         ctx.set_tuning("coarse_tpw", tpw)
+        ctx.set_tuning("coarse_fuse", fuse)
         total = 0.0
         for gate in gates:
             run = lambda: ctx.hier_peaks_async(0, nd, 0, nt, n, 0.0, 0, n, gate, idx.data_ptr(), val.data_ptr())
@@ -245,12 +251,12 @@ def _tune_coarse(device, n, nd, nt, band, reps, rng, calls=200):
                 ctx.sync()
                 times.append((time.perf_counter() - t0) * 1e3 / calls)
             total += sorted(times)[len(times) // 2]
-        ms[(lds, rpw, tpw)] = total
+        ms[(lds, rpw, tpw, fuse)] = total
     best = min(ms, key=ms.get)
     ctx.destroy()
     return {"op": "coarse", "n": n, "ndata": nd, "ntemplates": nt, "band": band,
-            "best": {"coarse_lds": best[0], "coarse_rpw": best[1], "coarse_tpw": best[2]},
-            "ms": {"%d:%dx%d" % k: round(v, 5) for k, v in ms.items()},
+            "best": {"coarse_lds": best[0], "coarse_rpw": best[1], "coarse_tpw": best[2], "coarse_fuse": best[3]},
+            "ms": {"%d:%dx%d%s" % (k[0], k[1], k[2], "f" if k[3] else ""): round(v, 5) for k, v in ms.items()},
             "timed": "ms per hierarchical call, back to back, summed over a gate passing none and one passing ~1%"}
 
 

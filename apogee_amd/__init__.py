@@ -315,12 +315,15 @@ class MatchedFilter:
         """
         req = self._check_run(binsize, window, data, templates)
         width, lo, hi, d0, nd, t0, nt = req
-        idx, val, cnt = self._dispatch(d0, nd, t0, nt, width, threshold, lo, hi, counts)
-        return _result(idx, val, raw, cnt if counts else None)
+        first, second, cnt = self._dispatch(d0, nd, t0, nt, width, threshold, lo, hi, counts, records=not raw)
+        body = (first, second) if raw else first
+        return body if not counts else (body, cnt)
 
-    def _dispatch(self, d0, nd, t0, nt, binsize, threshold, start, end, counts):
+    def _dispatch(self, d0, nd, t0, nt, binsize, threshold, start, end, counts, records=False):
+        """(index, value, counts) of a run, or with ``records`` (PEAK_DTYPE
+        records the library fills, None, counts)."""
         return self._gpu.peaks(d0, nd, t0, nt, binsize, float(threshold), start, end,
-                               counts=bool(counts))
+                               counts=bool(counts), records=PEAK_DTYPE if records else None)
 
     # ------------------------------------------------------------ series
     def _series_layout(self, series, starts, win_start, win_end, binsize, templates):

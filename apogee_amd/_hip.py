@@ -52,6 +52,9 @@ def _declare(lib):
     lib.mfh_flat_peaks.restype = ctypes.c_long
     lib.mfh_flat_peaks.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    _c_size, ctypes.c_float, _c_size, _c_size, _vp, _vp, _vp]
+    lib.mfh_flat_peaks_rec.restype = ctypes.c_long
+    lib.mfh_flat_peaks_rec.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _c_size, ctypes.c_float, _c_size, _c_size, _vp, _vp]
     lib.mfh_flat_peaks_async.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _c_size, ctypes.c_float, _c_size, _c_size, _vp, _vp]
     lib.mfh_hier_set_band.argtypes = [_vp, _c_size, ctypes.c_double]
@@ -59,6 +62,10 @@ def _declare(lib):
     lib.mfh_hier_peaks.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    _c_size, ctypes.c_float, _c_size, _c_size, ctypes.c_float,
                                    _vp, _vp, _vp, _vp]
+    lib.mfh_hier_peaks_rec.restype = ctypes.c_long
+    lib.mfh_hier_peaks_rec.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _c_size, ctypes.c_float, _c_size, _c_size, ctypes.c_float,
+                                       _vp, _vp, _vp]
     lib.mfh_hier_peaks_async.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _c_size, ctypes.c_float, _c_size, _c_size, ctypes.c_float,
                                          _vp, _vp, _vp]
@@ -143,6 +150,18 @@ def lib():
 
 def last_error():
     return lib().mfh_last_error().decode(errors="replace")
+
+
+def _record_dtype(dtype):
+    """The records layout the library writes: an int64 at byte 0 and a
+    complex64 at byte 8, 16 bytes in all (little-endian)."""
+    dt = np.dtype(dtype)
+    names = dt.names or ()
+    ok = (dt.itemsize == 16 and len(names) == 2 and dt.fields[names[0]][:2] == (np.dtype("<i8"), 0)
+          and dt.fields[names[1]][:2] == (np.dtype("<c8"), 8))
+    if not ok:
+        raise ValueError("records need an (int64, complex64) structured dtype of 16 bytes")
+    return dt
 
 
 def check(status, what="mfh"):
@@ -403,19 +422,28 @@ class Context:
         self._bound[kind] = tensor
 
     # ---------------------------------------------------------------- runs
-    def peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, counts=False):
+    def peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, counts=False, records=None):
         """Flat filter over rows [d0, d0+nd) x [t0, t0+nt); host results.
 
         Returns ``(index int64, value complex64)`` of shape (nd, nt, nbins), and
-        the per-pair crossing counts (int32, (nd, nt)) when ``counts``.
+        the per-pair crossing counts (int32, (nd, nt)) when ``counts``.  With
+        ``records`` (a structured dtype of an int64 and a complex64 field,
+        16 bytes) the first two are replaced by one array of that dtype,
+        filled by the library, and None.
         """
         nb = int(lib().mfh_nbins(self.n, int(binsize), int(start), int(end)))
+        cnt = np.empty((nd, nt), dtype=np.int32) if counts else None
+        args = (self.handle, int(d0), int(nd), int(t0), int(nt), int(binsize), float(threshold), int(start),
+                int(end))
+        if records is not None:
+            rec = np.empty((nd, nt, nb), dtype=_record_dtype(records))
+            check(lib().mfh_flat_peaks_rec(*args, _ptr(rec), _ptr(cnt) if cnt is not None else None),
+                  "mfh_flat_peaks_rec")
+            return rec, None, cnt
         idx = np.empty((nd, nt, nb), dtype=np.int64)
         val = np.empty((nd, nt, nb), dtype=np.complex64)
-        cnt = np.empty((nd, nt), dtype=np.int32) if counts else None
-        check(lib().mfh_flat_peaks(self.handle, int(d0), int(nd), int(t0), int(nt), int(binsize),
-                                   float(threshold), int(start), int(end), _ptr(idx), _ptr(val),
-                                   _ptr(cnt) if cnt is not None else None), "mfh_flat_peaks")
+        check(lib().mfh_flat_peaks(*args, _ptr(idx), _ptr(val), _ptr(cnt) if cnt is not None else None),
+              "mfh_flat_peaks")
         return idx, val, cnt
 
     def peaks_async(self, d0, nd, t0, nt, binsize, threshold, start, end, idx_ptr, val_ptr):
@@ -430,19 +458,29 @@ class Context:
         """Coarse band and normalisation (bank-wide fraction, or < 0 per template)."""
         check(lib().mfh_hier_set_band(self.handle, int(band), float(ref_fraction)), "mfh_hier_set_band")
 
-    def hier_peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, coarse_threshold, counts=False):
-        """Hierarchical run; returns (index, value, counts or None, refined pairs)."""
+    def hier_peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, coarse_threshold, counts=False,
+                   records=None):
+        """Hierarchical run; returns (index, value, counts or None, refined
+        pairs), or with ``records`` (as in :meth:`peaks`) (records, None,
+        counts or None, refined pairs)."""
         nb = int(lib().mfh_nbins(self.n, int(binsize), int(start), int(end)))
-        idx = np.empty((nd, nt, nb), dtype=np.int64)
-        val = np.empty((nd, nt, nb), dtype=np.complex64)
         cnt = np.empty((nd, nt), dtype=np.int32) if counts else None
         refined = ctypes.c_int64(0)
-        check(lib().mfh_hier_peaks(self.handle, int(d0), int(nd), int(t0), int(nt), int(binsize),
-                                   float(threshold), int(start), int(end), float(coarse_threshold),
-                                   _ptr(idx), _ptr(val), _ptr(cnt) if cnt is not None else None,
-                                   ctypes.byref(refined)), "mfh_hier_peaks")
+        args = (self.handle, int(d0), int(nd), int(t0), int(nt), int(binsize), float(threshold), int(start),
+                int(end), float(coarse_threshold))
+        if records is not None:
+            rec = np.empty((nd, nt, nb), dtype=_record_dtype(records))
+            check(lib().mfh_hier_peaks_rec(*args, _ptr(rec), _ptr(cnt) if cnt is not None else None,
+                                           ctypes.byref(refined)), "mfh_hier_peaks_rec")
+            first, second = rec, None
+        else:
+            idx = np.empty((nd, nt, nb), dtype=np.int64)
+            val = np.empty((nd, nt, nb), dtype=np.complex64)
+            check(lib().mfh_hier_peaks(*args, _ptr(idx), _ptr(val), _ptr(cnt) if cnt is not None else None,
+                                       ctypes.byref(refined)), "mfh_hier_peaks")
+            first, second = idx, val
         self.last_refinements = int(refined.value)
-        return idx, val, cnt, int(refined.value)
+        return first, second, cnt, int(refined.value)
 
     def hier_set_coarse_precision(self, precision, delta):
         """0 = FP32 screen, 1 = fp16 MFMA, 2 = bf16 MFMA with gate widening delta."""

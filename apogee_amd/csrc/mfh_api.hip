@@ -92,16 +92,16 @@ bool peak_buffer(Ctx *c, size_t cells, PeakBuf &b) {
   return true;
 }
 
-long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts) {
+long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts,
+                void *rec) {
   const size_t bi = cells * sizeof(int64_t), bytes = bi + cells * sizeof(cf);
   char *st = (char *)c->host_stage(bytes);
+  std::vector<char> pageable;
   if (st) {
     if (!hip_ok(hipMemcpyAsync(st, b.idx, bytes, hipMemcpyDeviceToHost, c->stream), "copy results") ||
         !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
       return -1;
-    memcpy(idx, st, bi);
-    memcpy(val_c64, st + bi, bytes - bi);
-  } else {
+  } else if (!rec) {
     // no page-locked staging to be had: copy straight into the caller's
     // (pageable) arrays, one copy each
     (void)hipGetLastError();
@@ -109,11 +109,34 @@ long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx,
         !hip_ok(hipMemcpyAsync(val_c64, b.val, bytes - bi, hipMemcpyDeviceToHost, c->stream), "copy val") ||
         !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
       return -1;
+  } else {
+    // records without page-locked staging: through a pageable host copy
+    (void)hipGetLastError();
+    pageable.resize(bytes);
+    st = pageable.data();
+    if (!hip_ok(hipMemcpyAsync(st, b.idx, bytes, hipMemcpyDeviceToHost, c->stream), "copy results") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+      return -1;
+  }
+  const int64_t *si = st ? reinterpret_cast<const int64_t *>(st) : idx;
+  if (rec) {
+    // one pass from the staging copy into {int64 index; complex64 value}
+    // records (16 bytes each)
+    const cf *sv = reinterpret_cast<const cf *>(st + bi);
+    char *r = static_cast<char *>(rec);
+    for (size_t i = 0; i < cells; i++) {
+      memcpy(r + 16 * i, si + i, 8);
+      memcpy(r + 16 * i + 8, sv + i, 8);
+    }
+  } else if (st) {
+    memcpy(idx, st, bi);
+    memcpy(val_c64, st + bi, bytes - bi);
+    si = idx;
   }
   long total = 0;
   for (size_t p = 0; p < cells / nb; p++) {
     int cnt = 0;
-    for (size_t k = 0; k < nb; k++) cnt += idx[p * nb + k] >= 0;
+    for (size_t k = 0; k < nb; k++) cnt += si[p * nb + k] >= 0;
     if (counts) counts[p] = cnt;
     total += cnt;
   }
@@ -420,11 +443,11 @@ extern "C" int mfh_flat_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt
   return mfh::launch_flat(c, c->n, r);
 }
 
-extern "C" long mfh_flat_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
-                               size_t ws, size_t we, int64_t *idx, void *val_c64, int32_t *counts) {
+static long flat_peaks_host(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws,
+                            size_t we, int64_t *idx, void *val_c64, void *rec, int32_t *counts) {
   if (!ctx) return fail("null context");
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  if (!idx || !val_c64) return fail("null output pointer");
+  if (!rec && (!idx || !val_c64)) return fail("null output pointer");
   const size_t nb = mfh_nbins(c->n, binsize, ws, we);
   if (binsize < 1) return fail("binsize must be >= 1");
   if (nb == 0) return fail("empty window (%zu, %zu)", ws, we);
@@ -433,9 +456,20 @@ extern "C" long mfh_flat_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, siz
   mfh::PeakBuf pb;
   if (!mfh::peak_buffer(c, cells, pb)) return -1;
   if (mfh_flat_peaks_async(ctx, d0, nd, t0, nt, binsize, threshold, ws, we, pb.idx, pb.val) < 0) return -1;
-  const long total = mfh::read_peaks(c, pb, cells, nb, idx, val_c64, counts);
+  const long total = mfh::read_peaks(c, pb, cells, nb, idx, val_c64, counts, rec);
   c->prof_collect();
   return total;
+}
+
+extern "C" long mfh_flat_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
+                               size_t ws, size_t we, int64_t *idx, void *val_c64, int32_t *counts) {
+  return flat_peaks_host(ctx, d0, nd, t0, nt, binsize, threshold, ws, we, idx, val_c64, nullptr, counts);
+}
+
+extern "C" long mfh_flat_peaks_rec(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
+                                   size_t ws, size_t we, void *rec, int32_t *counts) {
+  if (!rec) return fail("null output pointer");
+  return flat_peaks_host(ctx, d0, nd, t0, nt, binsize, threshold, ws, we, nullptr, nullptr, rec, counts);
 }
 
 extern "C" int mfh_set_tuning(mfh_ctx *ctx, const char *key, long value) {

@@ -472,12 +472,12 @@ extern "C" int mfh_hier_peaks_async(mfh_ctx *ctx, int d0, int nd, int t0, int nt
                               (mfh::cf *)dev_val_c64, host_refined);
 }
 
-extern "C" long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
-                               size_t ws, size_t we, float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
-                               int64_t *n_refined) {
+static long hier_peaks_host(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws,
+                            size_t we, float coarse_thr, int64_t *idx, void *val_c64, void *rec, int32_t *counts,
+                            int64_t *n_refined) {
   if (!ctx) return fail("null context");
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
-  if (!idx || !val_c64) return fail("null output pointer");
+  if (!rec && (!idx || !val_c64)) return fail("null output pointer");
   const size_t nb = mfh_nbins(c->n, binsize, ws, we);
   if (binsize < 1) return fail("binsize must be >= 1");
   if (nb == 0) return fail("empty window (%zu, %zu)", ws, we);
@@ -489,11 +489,26 @@ extern "C" long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, siz
   int refined = 0;
   if (mfh::hier_run_cached(c, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, pb.idx, pb.val, &refined) < 0)
     return -1;
-  const long total = mfh::read_peaks(c, pb, cells, nb, idx, val_c64, counts);
+  const long total = mfh::read_peaks(c, pb, cells, nb, idx, val_c64, counts, rec);
   if (total < 0) return -1;
   c->prof_collect();
   if (n_refined) *n_refined = refined;
   return total;
+}
+
+extern "C" long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
+                               size_t ws, size_t we, float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
+                               int64_t *n_refined) {
+  return hier_peaks_host(ctx, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, idx, val_c64, nullptr, counts,
+                         n_refined);
+}
+
+extern "C" long mfh_hier_peaks_rec(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t binsize, float threshold,
+                                   size_t ws, size_t we, float coarse_thr, void *rec, int32_t *counts,
+                                   int64_t *n_refined) {
+  if (!rec) return fail("null output pointer");
+  return hier_peaks_host(ctx, d0, nd, t0, nt, binsize, threshold, ws, we, coarse_thr, nullptr, nullptr, rec, counts,
+                         n_refined);
 }
 
 // ---------------------------------------------------------------------------

@@ -142,7 +142,8 @@ bool peak_buffer(Ctx *c, size_t cells, PeakBuf &b);
 // copy into the pinned staging buffer, then host copies -- after waiting for
 // the context's stream; counts[p] (if set) = bins set of pair p, nb bins per
 // pair.  Returns the total of set bins, or -1 (message set).
-long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts);
+long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts,
+                void *rec = nullptr);
 
 // One flat launch: pairs (data rows x template rows) from the given bases.
 struct FlatRun {

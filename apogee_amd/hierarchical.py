@@ -26,7 +26,7 @@ cost wins.  Lengths the table lacks fall back to the band n/8 (at most
 """
 import numpy as np
 
-from . import MatchedFilter, _host_view
+from . import PEAK_DTYPE, MatchedFilter, _host_view
 
 #: Coarse-screen precisions: name -> (code of mfh_hier_set_coarse_precision,
 #: default gate widening).  The widening bounds the relative error of the
@@ -251,11 +251,12 @@ class HierarchicalFilter(MatchedFilter):
         self._cal = (key, out)
         return out
 
-    def _dispatch(self, d0, nd, t0, nt, binsize, threshold, start, end, counts):
+    def _dispatch(self, d0, nd, t0, nt, binsize, threshold, start, end, counts, records=False):
         band, f, thr = self._calibration()
         self._apply_band(band, f)
         idx, val, cnt, refined = self._gpu.hier_peaks(d0, nd, t0, nt, binsize, float(threshold),
-                                                      start, end, thr, counts=bool(counts))
+                                                      start, end, thr, counts=bool(counts),
+                                                      records=PEAK_DTYPE if records else None)
         self._pairs += nd * nt
         self._trig += int(refined)
         return idx, val, cnt

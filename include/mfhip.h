@@ -135,6 +135,15 @@ long mfh_flat_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
                     size_t binsize, float threshold, size_t ws, size_t we,
                     int64_t *idx, void *val_c64, int32_t *counts);
 
+/* As mfh_flat_peaks, with the results as records: rec receives nd*nt*nbins
+ * records {int64 index; float re, im} (16 bytes each, the PEAK_DTYPE layout
+ * of the Python surface), filled in one pass over the page-locked readback.
+ * Replaces: the record assembly of _format_result
+ * (python/matchedfilter/__init__.py:155-173) after Context.peaks. */
+long mfh_flat_peaks_rec(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
+                        size_t binsize, float threshold, size_t ws, size_t we,
+                        void *rec, int32_t *counts);
+
 /* Same with device output pointers; queued on the context stream, no sync.
  * Returns 0 or < 0.  Replaces: the fusedTierB dispatch recorded by
  * Context._make_batch (python/matchedfilter/_vkcompute.py:923-960). */
@@ -181,6 +190,15 @@ long mfh_hier_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
                     size_t binsize, float threshold, size_t ws, size_t we,
                     float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
                     int64_t *n_refined);
+
+/* As mfh_hier_peaks, with the results as 16-byte records (see
+ * mfh_flat_peaks_rec).
+ * Replaces: the record assembly of _format_result
+ * (python/matchedfilter/__init__.py:155-173) after Context.hier_peaks. */
+long mfh_hier_peaks_rec(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
+                        size_t binsize, float threshold, size_t ws, size_t we,
+                        float coarse_thr, void *rec, int32_t *counts,
+                        int64_t *n_refined);
 
 /* mfh_hier_peaks with device outputs, queued on the stream; if host_refined
  * is not NULL it receives the survivor count once the stream reaches that

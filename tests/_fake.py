@@ -29,12 +29,22 @@ class FakeContext:
         dst[row0:row0 + a.shape[0]] = a
         FakeContext.calls.append(("set", kind, row0, a.shape[0]))
 
-    def peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, counts=False):
+    @staticmethod
+    def _as_records(idx, val, records):
+        """The library's records option: one structured array, then None."""
+        if records is None:
+            return idx, val
+        rec = np.empty(idx.shape, records)
+        rec[rec.dtype.names[0]] = idx
+        rec[rec.dtype.names[1]] = val
+        return rec, None
+
+    def peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, counts=False, records=None):
         FakeContext.calls.append(("peaks", d0, nd, t0, nt, binsize, threshold, start, end))
         idx, val, _ = O.flat_peaks(self.D[d0:d0 + nd], self.T[t0:t0 + nt], binsize, threshold,
                                    (start, end))
         cnt = (idx >= 0).sum(axis=2).astype(np.int32) if counts else None
-        return idx, val.astype(np.complex64), cnt
+        return self._as_records(idx, val.astype(np.complex64), records) + (cnt,)
 
     def hier_set_band(self, band, ref_fraction=-1.0):
         self.band, self.ref_f = band, ref_fraction
@@ -43,7 +53,7 @@ class FakeContext:
         self.coarse_precision = (precision, delta)
 
     def hier_peaks(self, d0, nd, t0, nt, binsize, threshold, start, end, coarse_threshold,
-                   counts=False):
+                   counts=False, records=None):
         FakeContext.calls.append(("hier", d0, nd, t0, nt, coarse_threshold))
         T = self.T[t0:t0 + nt]
         if self.ref_f >= 0:
@@ -59,7 +69,7 @@ class FakeContext:
                                           binsize, threshold, (start, end), coarse_max=cmax)
         cnt = (idx >= 0).sum(axis=2).astype(np.int32) if counts else None
         self.last_refinements = int(fired.sum())
-        return idx, val.astype(np.complex64), cnt, int(fired.sum())
+        return self._as_records(idx, val.astype(np.complex64), records) + (cnt, int(fired.sum()))
 
     def run_series(self, series, starts, win_start, win_end, t0, nt, binsize, threshold,
                    coarse_threshold=None):

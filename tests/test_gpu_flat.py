@@ -557,3 +557,35 @@ def test_lengths_beyond_the_reference_gpu_set(n, gpu_device):
         got = f.run(binsize=binsize, threshold=0.0, **kw)
         assert_peaks_match(D, T, got["index"], got["value"], binsize, 0.0, window,
                            context="n=%d binsize=%d" % (n, binsize))
+
+
+def test_records_equal_the_raw_arrays(gpu_device):
+    """run() has the library fill PEAK_DTYPE records directly
+    (mfh_flat_peaks_rec, mfh_hier_peaks_rec); they must equal the raw
+    (index, value) arrays of the same run bit for bit, counts included:
+    flat with ragged bins over a window and with a threshold, hierarchical
+    with one bin."""
+    rng = np.random.default_rng(77)
+    n, nd, nt = 4096, 3, 37
+    D, T = cnoise(rng, (nd, n)), cnoise(rng, (nt, n))
+    D[1] += 4.0 * T[5]
+    f = mf.MatchedFilter(n, nd, nt, device="gpu")
+    f.set_data(D)
+    f.set_templates(T)
+    for kw in ({"binsize": 300, "threshold": 0.0, "window": (17, 3999)}, {"binsize": n, "threshold": 40.0}):
+        rec, cnt = f.run(counts=True, **kw)
+        (idx, val), cnt_raw = f.run(counts=True, raw=True, **kw)
+        assert rec.dtype == mf.PEAK_DTYPE and rec.shape == idx.shape, kw
+        np.testing.assert_array_equal(rec["index"], idx, err_msg=str(kw))
+        np.testing.assert_array_equal(rec["value"], val, err_msg=str(kw))
+        np.testing.assert_array_equal(cnt, cnt_raw, err_msg=str(kw))
+        np.testing.assert_array_equal(cnt, (idx >= 0).sum(axis=2), err_msg=str(kw))
+    h = mf.HierarchicalFilter(n, nd, nt, snr=5.5, fd=1e-2, band=512, device="gpu")
+    h.set_coarse_threshold(3.0)
+    h.set_data(D)
+    h.set_templates(T)
+    rec, cnt = h.run(binsize=n, threshold=4.5, counts=True)
+    (idx, val), cnt_raw = h.run(binsize=n, threshold=4.5, counts=True, raw=True)
+    np.testing.assert_array_equal(rec["index"], idx)
+    np.testing.assert_array_equal(rec["value"], val)
+    np.testing.assert_array_equal(cnt, cnt_raw)

@@ -55,7 +55,8 @@ def test_supported_lengths_query():
 
 
 def test_integration_binding_is_the_tested_shim():
-    """INTEGRATION.md shows tests/hip_context.py verbatim, the binding that
+    """The integration notes at the repository root show
+    tests/hip_context.py verbatim, the binding that
     tests/test_gpu_context_shim.py runs on the GPU."""
     doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     blocks = re.findall(r"```python\n(.*?)```", doc, flags=re.S)

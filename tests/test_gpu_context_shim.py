@@ -1,4 +1,4 @@
-"""The reference-side binding of INTEGRATION.md (tests/hip_context.py: the
+"""The reference-side binding (tests/hip_context.py: the
 reference's Context protocol of python/matchedfilter/_vkcompute.py over
 libmfhip.so through ctypes) run against the FP64 oracle and this package's
 own classes.  Covers ``peaks`` (int32 index), ``hier_peaks`` with

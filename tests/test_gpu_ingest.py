@@ -6,8 +6,8 @@ after the producer's pending work on the device by the DLPack stream
 hand-over.  Results must equal the host-ingest results bit for bit, the
 library must never write the producer's buffer, and a later update of the
 buffer takes effect when the bank is set again.  The reference refuses
-device arrays (python/matchedfilter/__init__.py:103-131); the north star asks
-for them to be used in place.
+device arrays (python/matchedfilter/__init__.py:103-131); here they are used
+in place.
 """
 import numpy as np
 import pytest

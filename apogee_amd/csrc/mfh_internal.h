@@ -139,9 +139,11 @@ struct PeakBuf {
 };
 bool peak_buffer(Ctx *c, size_t cells, PeakBuf &b);
 // Copy `cells` results into the caller's host arrays -- one device-to-host
-// copy into the pinned staging buffer, then host copies -- after waiting for
-// the context's stream; counts[p] (if set) = bins set of pair p, nb bins per
-// pair.  Returns the total of set bins, or -1 (message set).
+// copy into the pinned staging buffer, then host copies, or with `rec` one
+// pass writing 16-byte {int64 index; complex64 value} records there (idx and
+// val unused) -- after waiting for the context's stream; counts[p] (if set) =
+// bins set of pair p, nb bins per pair.  Returns the total of set bins, or -1
+// (message set).
 long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts,
                 void *rec = nullptr);
 

@@ -548,7 +548,8 @@ __global__ void __launch_bounds__(256) coarse_mfma_kernel(const cf *data, long l
 // own survivors with all 256 threads (one FN = 4096 transform at a time), so
 // no survivor list leaves the workgroup and no second launch waits on the
 // first.  The number of survivors still reaches g.count (one atomic per
-// workgroup that has any) for the caller's refinement count.  The screen's
+// workgroup that has any, after its refines) for the caller's refinement
+// count.  The screen's
 // LDS (constants, exponents, fragments) and the refine's (exchange buffer,
 // bin keys) share one dynamic allocation; the refine starts after a barrier
 // that ends every read of the screen's.  A workgroup's refines run one after
@@ -580,11 +581,7 @@ coarse_refine_kernel(const cf *data, long long data_stride, const cf *ctm, const
   for (int k = 0; k < R; k++) total += __popcll(fired[k]);
   if (lane == 0) wave_cnt[w] = total;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const int sum = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
-    wg_n = sum;
-    if (sum) atomicAdd(g.count, sum);
-  }
+  if (threadIdx.x == 0) wg_n = wave_cnt[0] + wave_cnt[1] + wave_cnt[2] + wave_cnt[3];
   int base = 0;
   for (int i = 0; i < w; i++) base += wave_cnt[i];
   const int cnt = t1 - t0;
@@ -626,6 +623,9 @@ coarse_refine_kernel(const cf *data, long long data_stride, const cf *ctm, const
     select_and_write<FN, kOneBin, false>(fa, 1, v, t, lane, bins_all + (i & 1), bins_all + ((i + 1) & 1), true, pr.x,
                                          pr.y, 0, 0);
   }
+  // the count last: issued before the refines, every workgroup's atomic on
+  // the one counter would hold them back until it completed
+  if (threadIdx.x == 0) atomicAdd(g.count, ns);
 }
 
 // exp(+2 pi i k / L), k < L, FP64-generated

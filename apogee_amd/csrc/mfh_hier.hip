@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "mfh_dft.h"
+#include "mfh_flat.h"
 #include "mfh_internal.h"
 #include "mfhip.h"
 
@@ -97,6 +98,78 @@ __global__ void gate_kernel(const int64_t *cidx, const cf *cval, int nd, int nt,
   }
 }
 
+// Gate and refine in one launch after the FP32 screen (refine length FN, one
+// transform per 256-thread workgroup): each workgroup takes a tile of
+// kGateRows data rows x kGateTemplates templates, decides their gates in
+// parallel by gate_kernel's rule from the screen's results, writes -1 / 0
+// into every bin of the dismissed pairs, clears the next call's counter,
+// runs the flat kernel's per-pair path on each survivor in turn, and adds
+// its survivors to this call's counter (one atomic per workgroup that has
+// any).  Replaces gate_kernel, the survivor list and the list-driven refine
+// launch.  A dense gate gives every workgroup at most 16 refines, whose rows
+// and templates are each read four times in a row.
+constexpr int kGateRows = 4, kGateTemplates = 4;
+template <int FN, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+gate_refine_kernel(const int64_t *cidx, const cf *cval, int nd, int nt, float thr, int *count, int *other,
+                   FlatArgs fa) {
+  using S = FlatShape<FN>;
+  static_assert(S::WG == 256 && S::SLOTS == 1 && S::TPT == 256, "one transform per 256-thread workgroup");
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int2 surv[kGateRows * kGateTemplates];
+  __shared__ int ns_sh;
+  const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *other = 0;  // the next call's counter
+  const int nb = (int)fa.nb;
+  if (threadIdx.x < 64) {
+    const int d = blockIdx.y * kGateRows + lane / kGateTemplates;
+    const int tt = blockIdx.x * kGateTemplates + lane % kGateTemplates;
+    const bool inside = lane < kGateRows * kGateTemplates && d < nd && tt < nt;
+    const long long p = (long long)d * nt + tt;
+    bool fire = false;
+    if (inside && cidx[p] >= 0) {
+      const cf z = cval[p];
+      const float m2 = __builtin_fmaf(z.x, z.x, z.y * z.y);
+      fire = sqrtf(m2) >= thr;
+    }
+    const unsigned long long ballot = __ballot(fire);
+    if (fire) surv[__popcll(ballot & ((1ull << lane) - 1ull))] = int2{d, tt};
+    if (inside && !fire) {
+      for (int b = 0; b < nb; b++) {
+        fa.out_idx[p * fa.ostride + b] = -1;
+        fa.out_val[p * fa.ostride + b] = cf{0.f, 0.f};
+      }
+    }
+    if (lane == 0) ns_sh = __popcll(ballot);
+  }
+  __syncthreads();
+  const int ns = ns_sh;
+  if (ns == 0) return;  // workgroup-uniform
+
+  cf *xch = reinterpret_cast<cf *>(smem);
+  uint64_t *bins_all = reinterpret_cast<uint64_t *>(xch + S::XSTRIDE);
+  cf wfirst[15];
+  first_twiddles<FN>(wfirst, (int)threadIdx.x, fa.twiddle);
+  for (int i = threadIdx.x; i < 2 * nb; i += 256) bins_all[i] = 0ull;
+  __syncthreads();
+  for (int i = 0; i < ns; i++) {
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const int2 pr = surv[i];
+    cf dv[16], tv[16], v[16];
+    load_row_uniform<FN>(dv, uniform_ptr(fa.data + (long long)pr.x * fa.data_stride), t);
+    load_row_uniform<FN>(tv, uniform_ptr(fa.tmpl + (long long)pr.y * fa.tmpl_stride), t);
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = cmul_conj(dv[u], tv[u]);
+    inverse_fft<FN, true>(v, t, xch, fa.twiddle, wfirst);
+    select_and_write<FN, MODE, false>(fa, nb, v, t, lane, bins_all + (i & 1) * nb, bins_all + ((i + 1) & 1) * nb,
+                                      true, pr.x, pr.y, 0, 0);
+  }
+  // the count last: issued at the start, every workgroup's atomic on the one
+  // counter would hold its refines back until it completed
+  if (threadIdx.x == 0) atomicAdd(count, ns);
+}
+
 // Clears `n` ints (n <= 64), one workgroup: a kernel rather than a memset so
 // that a captured sequence holds no memset node.
 __global__ void zero_ints_kernel(int *p, int n) {
@@ -138,6 +211,53 @@ int *hier_counter(Ctx *c, int parity) {
     if (!hip_ok(hipMemsetAsync(cnt, 0, 2 * sizeof(int), c->stream), "hipMemsetAsync")) return nullptr;
   }
   return cnt + (parity & 1);
+}
+
+// The refine's launch description (everything but the list fields): the
+// run's data rows and templates, its window, bins and threshold, its outputs.
+static FlatRun refine_base(Ctx *c, const cf *drows, int t0, int nd, int nt, size_t binsize, float threshold,
+                           size_t ws, size_t we, int64_t *idx, cf *val) {
+  FlatRun rr;
+  rr.data = drows;
+  rr.tmpl = c->tmpl + (size_t)t0 * c->n;
+  rr.data_stride = (long long)c->n;
+  rr.tmpl_stride = (long long)c->n;
+  rr.nd = nd;
+  rr.nt = nt;
+  rr.binsize = binsize;
+  rr.threshold = threshold;
+  rr.ws = ws;
+  rr.we = we;
+  rr.out_idx = idx;
+  rr.out_val = val;
+  return rr;
+}
+
+// After the FP32 screen: gate and refine in one launch (tuning key
+// coarse_fuse), 4 x 4 pairs per workgroup, at n = 4096 with one bin or with the
+// window and bins on 64-lag boundaries (the general bin form of the kernel
+// would spill registers; it keeps the separate launches).
+static bool gate_refine_fits(const Ctx *c, size_t n, size_t nb, size_t ws, size_t we, size_t binsize) {
+  return c->coarse_fuse && n == 4096 && (nb == 1 || (ws % 64 == 0 && we % 64 == 0 && binsize % 64 == 0));
+}
+
+static int launch_gate_refine(Ctx *c, const int64_t *cidx, const cf *cval, int nd, int nt, float thr, int *count,
+                              int *other, const FlatRun &rr) {
+  constexpr int FN = 4096;
+  FlatArgs fa;
+  if (flat_fill_args(c, FN, rr, fa) < 0) return -1;
+  const int nb = (int)fa.nb;
+  if (!gate_refine_fits(c, FN, nb, fa.ws, fa.we, fa.binsize))
+    return fail("the one-launch gate and refine does not apply here");
+  using Kern = decltype(&gate_refine_kernel<FN, kOneBin>);
+  const Kern kern = nb == 1 ? gate_refine_kernel<FN, kOneBin> : gate_refine_kernel<FN, kAligned>;
+  const dim3 grid((unsigned)((nt + kGateTemplates - 1) / kGateTemplates), (unsigned)((nd + kGateRows - 1) / kGateRows));
+  if (grid.y > 65535) return fail("too many data rows per launch (%d)", nd);
+  const size_t smem = (size_t)FlatShape<FN>::XSTRIDE * sizeof(cf) + 2 * (size_t)nb * sizeof(uint64_t);
+  if (!lds_limit((const void *)kern, c->dev, smem)) return -1;
+  Ctx::ProfScope ps(c, "refine");
+  hipLaunchKernelGGL(kern, grid, dim3(256), smem, c->stream, cidx, cval, nd, nt, thr, count, other, fa);
+  return hip_ok(hipGetLastError(), "gate_refine_kernel") ? 0 : -1;
 }
 
 int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float threshold, size_t ws, size_t we,
@@ -203,19 +323,7 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
       // one launch: each workgroup of the screen refines its own survivors
       // (the count still goes to this call's counter; no list, no second
       // launch)
-      FlatRun rr;
-      rr.data = drows;
-      rr.tmpl = c->tmpl + (size_t)t0 * n;
-      rr.data_stride = (long long)n;
-      rr.tmpl_stride = (long long)n;
-      rr.nd = nd;
-      rr.nt = nt;
-      rr.binsize = binsize;
-      rr.threshold = threshold;
-      rr.ws = ws;
-      rr.we = we;
-      rr.out_idx = idx;
-      rr.out_val = val;
+      const FlatRun rr = refine_base(c, drows, t0, nd, nt, binsize, threshold, ws, we, idx, val);
       if (launch_coarse_refine(c, drows, (long long)n, t0, nd, nt, (int)cs, (int)ce, c->coarse_prec == 2, g, rr) < 0)
         return -1;
       if (!c->capturing && !hs) c->hier_parity ^= 1;
@@ -245,6 +353,17 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   cr.zero = count;  // the coarse launch clears the survivor counter
   if (!mfma) {
     if (launch_flat(c, m, cr) < 0) return -1;
+    if (gate_refine_fits(c, n, nb, ws, we, binsize)) {
+      // the gate and the refine in one launch; the screen above cleared this
+      // call's counter, the launch clears the next call's
+      const FlatRun rr = refine_base(c, drows, t0, nd, nt, binsize, threshold, ws, we, idx, val);
+      if (launch_gate_refine(c, cidx, cval, nd, nt, coarse_thr, count, other, rr) < 0) return -1;
+      if (!c->capturing && !hs) c->hier_parity ^= 1;
+      if (count_out &&
+          !hip_ok(hipMemcpyAsync(count_out, count, sizeof(int), hipMemcpyDeviceToHost, c->stream), "copy count"))
+        return -1;
+      return 0;
+    }
     Ctx::ProfScope ps(c, "gate");
     hipLaunchKernelGGL(gate_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, c->stream, cidx, cval, nd,
                        nt, coarse_thr, (int)nb, idx, val, list, count, other);
@@ -255,19 +374,7 @@ int hier_run(Ctx *c, int d0, int nd, int t0, int nt, size_t binsize, float thres
   // leave the context's pair untouched)
   if (!c->capturing && !hs) c->hier_parity ^= 1;
 
-  FlatRun rr;
-  rr.data = drows;
-  rr.tmpl = c->tmpl + (size_t)t0 * n;
-  rr.data_stride = (long long)n;
-  rr.tmpl_stride = (long long)n;
-  rr.nd = nd;
-  rr.nt = nt;
-  rr.binsize = binsize;
-  rr.threshold = threshold;
-  rr.ws = ws;
-  rr.we = we;
-  rr.out_idx = idx;
-  rr.out_val = val;
+  FlatRun rr = refine_base(c, drows, t0, nd, nt, binsize, threshold, ws, we, idx, val);
   rr.list = list;
   rr.list_count = count;
   rr.list_cap = (int)pairs;

@@ -391,17 +391,20 @@ def test_refine_grid_sized_from_a_stale_count_drains_every_survivor(prec, gpu_de
             assert_one_sided(ref, out, ctx)
 
 
-@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32"])
 def test_one_launch_screen_and_refine_matches_two_launches(prec, gpu_device):
-    """At n = 4096 with one bin the matrix-core screen refines its own
-    survivors in the same launch (tuning key coarse_fuse).  The two-launch
-    path (screen, survivor list, list-driven refine) runs the same
-    arithmetic in another order of launches, so every output must be
-    bit-identical, and so must the refinement count: with the automatic gate
-    (a few survivors), a gate a quarter of the pairs pass, and every pair
-    refined; over splits of rows and templates per wave (data rows not a
-    multiple of a workgroup's, templates not of a wave's), with the whole
-    transform and with a lag window."""
+    """At n = 4096 the hierarchical call runs in fewer launches (tuning key
+    coarse_fuse): with the matrix-core screen and one bin, the screen's
+    workgroups refine their own survivors; with the FP32 screen, one launch
+    gates and refines after the screen (one bin, or bins on 64-lag
+    boundaries).  The separate launches (screen, gate, survivor list,
+    list-driven refine) run the same arithmetic in another order of launches,
+    so every output must be bit-identical, and so must the refinement count:
+    with the automatic gate (a few survivors), a gate a quarter of the pairs
+    pass, and every pair refined; over splits of rows and templates per wave
+    (data rows not a multiple of a workgroup's, templates not of a wave's),
+    one bin and 64-lag bins, the whole transform, an aligned window and an
+    unaligned one (which keeps the separate launches for bins)."""
     n, nd, nt = 4096, 12, 200
     power, D, H = loud_case(n, nd, nt, seed=43, snr=12.0)
     hier = mf.HierarchicalFilter(n, nd, nt, snr=5.5, fd=1e-2, band=512, device="gpu")
@@ -413,23 +416,26 @@ def test_one_launch_screen_and_refine_matches_two_launches(prec, gpu_device):
     ctx.set_tuning("coarse_lds", 1)
     cmax = np.sort(hier.coarse_statistic().ravel())
     gates = (None, float(cmax[int(0.75 * cmax.size)]), 0.0)
-    for rpw, tpw in ((1, 4), (2, 2), (1, 16), (2, 16), (1, 1)):
+    splits = ((1, 4), (2, 2), (1, 16), (2, 16), (1, 1)) if prec != "fp32" else ((1, 4),)
+    for rpw, tpw in splits:
         ctx.set_tuning("coarse_rpw", rpw)
         ctx.set_tuning("coarse_tpw", tpw)
         for gate in gates:
             hier.set_coarse_threshold(gate)
-            for window in (None, (100, 3900)):
-                got = []
-                for fuse in (1, 0):
-                    ctx.set_tuning("coarse_fuse", fuse)
-                    before = hier.stats[1]
-                    out = hier.run(binsize=n, threshold=4.5, window=window)
-                    got.append((out, hier.stats[1] - before))
-                (a, ra), (b, rb) = got
-                label = "%s, %d x %d per wave, gate %s, window %s" % (prec, rpw, tpw, gate, window)
-                print(label, "refined", ra, flush=True)
-                assert ra == rb, label
-                if gate == 0.0:
-                    assert ra == nd * nt, label
-                np.testing.assert_array_equal(a["index"], b["index"], err_msg=label)
-                np.testing.assert_array_equal(a["value"], b["value"], err_msg=label)
+            for binsize in (n, 512):
+                for window in (None, (128, 3968), (100, 3900)):
+                    got = []
+                    for fuse in (1, 0):
+                        ctx.set_tuning("coarse_fuse", fuse)
+                        before = hier.stats[1]
+                        out = hier.run(binsize=binsize, threshold=4.5, window=window)
+                        got.append((out, hier.stats[1] - before))
+                    (a, ra), (b, rb) = got
+                    label = "%s, %d x %d per wave, gate %s, binsize %d, window %s" % (
+                        prec, rpw, tpw, gate, binsize, window)
+                    print(label, "refined", ra, flush=True)
+                    assert ra == rb, label
+                    if gate == 0.0:
+                        assert ra == nd * nt, label
+                    np.testing.assert_array_equal(a["index"], b["index"], err_msg=label)
+                    np.testing.assert_array_equal(a["value"], b["value"], err_msg=label)

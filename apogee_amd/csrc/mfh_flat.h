@@ -225,11 +225,27 @@ __device__ __forceinline__ bool full_window_bins(const cf *v, int t, int lane, f
 // the lanes that hold it.  Returns false when the wave holds a NaN (bits
 // above +inf): the caller then runs the per-sample scan, which never selects
 // one.
-template <int TPT>
-__device__ __forceinline__ bool one_bin_wave_key(const cf *v, int t, float seed, uint64_t &wkey) {
+//
+// WINDOWED: the window [ws, we) need not be the whole transform, but both
+// ends are multiples of 64 lags.  A wave's 64 lags at one u are then
+// consecutive and all inside or all outside the window (a wave-uniform
+// test), and the lags outside count as |z|^2 = 0, which never beats a seed
+// >= 0 -- the per-sample scan's result, without its per-sample tests.
+template <int TPT, bool WINDOWED = false>
+__device__ __forceinline__ bool one_bin_wave_key(const cf *v, int t, float seed, uint64_t &wkey, uint32_t ws = 0,
+                                                 uint32_t we = 0) {
   uint32_t mb[16];
 #pragma unroll
   for (int u = 0; u < 16; u++) mb[u] = __float_as_uint(__builtin_fmaf(v[u].x, v[u].x, v[u].y * v[u].y));
+  if constexpr (WINDOWED) {
+    static_assert(TPT % 64 == 0, "a wave's lags at one u are 64 consecutive lags");
+    const uint32_t kw = (uint32_t)__builtin_amdgcn_readfirstlane(t) & ~63u;  // the wave's first lag at u = 0
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const uint32_t k = kw + (uint32_t)(u * TPT);
+      if (!(k >= ws && k < we)) mb[u] = 0u;  // wave-uniform
+    }
+  }
   uint32_t m = mb[0];
 #pragma unroll
   for (int u = 1; u < 16; u++) m = max(m, mb[u]);
@@ -445,6 +461,8 @@ __device__ __forceinline__ void select_and_write(const FlatArgs &a, int nb, cons
     bool have = false;
     if constexpr (TPT >= 64) {
       if (a.ws == 0 && a.we == (uint32_t)N) have = one_bin_wave_key<TPT>(v, t, a.seed, wkey);
+      else if ((a.ws & 63u) == 0 && (a.we & 63u) == 0)
+        have = one_bin_wave_key<TPT, true>(v, t, a.seed, wkey, a.ws, a.we);
     }
     if (!have) {
       // per-sample scan: strict, so the lowest lag keeps a tie

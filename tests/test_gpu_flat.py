@@ -589,3 +589,33 @@ def test_records_equal_the_raw_arrays(gpu_device):
     np.testing.assert_array_equal(rec["index"], idx)
     np.testing.assert_array_equal(rec["value"], val)
     np.testing.assert_array_equal(cnt, cnt_raw)
+
+
+@pytest.mark.parametrize("n", [1024, 4096])
+def test_one_bin_over_aligned_windows(n, gpu_device):
+    """One bin over a window whose ends are multiples of 64 lags takes the
+    wave-key selection (the wave's largest |z|^2, lowest lag first, lags
+    outside the window counted as zero) instead of the per-sample scan: the
+    peaks must equal the FP64 oracle's, with loud peaks planted at each
+    window's first lag, just before it, at its last lag and just after it,
+    with and without a threshold."""
+    rng = np.random.default_rng(91 + n)
+    nd, nt = 3, 21
+    D, T = cnoise(rng, (nd, n)), cnoise(rng, (nt, n))
+    f_ = np.arange(n)
+    windows = [(n // 4, 3 * n // 4), (64, n - 64), (0, n // 2), (n // 2, n), (n - 64, n)]
+    j = 0
+    for ws, we in windows:
+        for lag in (ws, ws - 1, we - 1, we):
+            if 0 <= lag < n:
+                # D_d += a T_t exp(-2 pi i f lag / n): z of (d, t) peaks at `lag`
+                D[j % nd] += (3.0 * T[(5 * j) % nt] * np.exp(-2j * np.pi * f_ * lag / n)).astype(np.complex64)
+                j += 1
+    f = mf.MatchedFilter(n, nd, nt, device="gpu")
+    f.set_data(D)
+    f.set_templates(T)
+    for ws, we in windows:
+        for thr in (0.0, 3.0):
+            got = f.run(binsize=n, threshold=thr, window=(ws, we))
+            assert_peaks_match(D, T, got["index"], got["value"], n, thr, window=(ws, we),
+                               context="n %d window %s threshold %s" % (n, (ws, we), thr))

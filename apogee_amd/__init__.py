@@ -376,13 +376,17 @@ class MatchedFilter:
                                                          binsize, templates)
         self._has_data = False
         self._books["data"].forget()
-        idx, val = self._dispatch_series(ser, layout, width, float(threshold), t0, nt)
-        return _result(idx, val, raw, None)
+        first, second = self._dispatch_series(ser, layout, width, float(threshold), t0, nt,
+                                              records=not raw)
+        return (first, second) if raw else first
 
-    def _dispatch_series(self, ser, layout, binsize, threshold, t0, nt):
-        idx, val, _ = self._gpu.run_series(ser, layout.starts, layout.low, layout.high, t0, nt,
-                                           binsize, threshold)
-        return idx, val
+    def _dispatch_series(self, ser, layout, binsize, threshold, t0, nt, records=False):
+        """(index, value) of a series run, or with ``records`` (PEAK_DTYPE
+        records the library fills, None)."""
+        first, second, _ = self._gpu.run_series(ser, layout.starts, layout.low, layout.high, t0, nt,
+                                                binsize, threshold,
+                                                records=PEAK_DTYPE if records else None)
+        return first, second
 
     # ---------------------------------------------------------- memory
     def empty_shared(self, shape, dtype=np.complex64):

@@ -69,13 +69,18 @@ def _declare(lib):
     lib.mfh_hier_peaks_async.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _c_size, ctypes.c_float, _c_size, _c_size, ctypes.c_float,
                                          _vp, _vp, _vp]
-    for name in ("mfh_run_series", "mfh_hier_run_series"):
+    for name in ("mfh_run_series", "mfh_hier_run_series", "mfh_run_series_rec", "mfh_hier_run_series_rec"):
         getattr(lib, name).restype = ctypes.c_long
     lib.mfh_run_series.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, _c_size, ctypes.c_float, _vp, _vp, _vp]
     lib.mfh_hier_run_series.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, _c_size, ctypes.c_float,
                                         ctypes.c_float, _vp, _vp, _vp, _vp]
+    lib.mfh_run_series_rec.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, _c_size, ctypes.c_float, _vp, _vp]
+    lib.mfh_hier_run_series_rec.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, _c_size, ctypes.c_float,
+                                            ctypes.c_float, _vp, _vp, _vp]
     lib.mfh_series_forward.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int]
     lib.mfh_hier_set_coarse_precision.argtypes = [_vp, ctypes.c_int, ctypes.c_float]
     lib.mfh_hier_coarse_max.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -504,9 +509,10 @@ class Context:
 
     # ------------------------------------------------------------ series
     def run_series(self, series, starts, win_start, win_end, t0, nt, binsize, threshold,
-                   coarse_threshold=None):
+                   coarse_threshold=None, records=None):
         """Block spectra of a series on the device, then the flat (or, with a
-        coarse threshold, hierarchical) filter; returns (index, value, refined)."""
+        coarse threshold, hierarchical) filter; returns (index, value, refined),
+        or (records, None, refined) when `records` names the record dtype."""
         view = self.borrow(series) if _device_pointer(series) is not None else None
         if view is not None:
             sp, ns, on_dev = ctypes.c_void_p(view.ptr), int(view.shape[0]), 1
@@ -517,9 +523,22 @@ class Context:
         we = np.ascontiguousarray(win_end, dtype=np.uintp)
         nblk = int(st.size)
         nb = int(lib().mfh_nbins(self.n, int(binsize), int(ws[0]), int(we[0])))
+        refined = ctypes.c_int64(0)
+        if records is not None:
+            rec = np.empty((nblk, nt, nb), dtype=_record_dtype(records))
+            if coarse_threshold is None:
+                check(lib().mfh_run_series_rec(self.handle, sp, ns, on_dev, _ptr(st), _ptr(ws), _ptr(we), nblk,
+                                               int(t0), int(nt), int(binsize), float(threshold), _ptr(rec),
+                                               None), "mfh_run_series_rec")
+            else:
+                check(lib().mfh_hier_run_series_rec(self.handle, sp, ns, on_dev, _ptr(st), _ptr(ws), _ptr(we),
+                                                    nblk, int(t0), int(nt), int(binsize), float(threshold),
+                                                    float(coarse_threshold), _ptr(rec), None,
+                                                    ctypes.byref(refined)), "mfh_hier_run_series_rec")
+            self.last_refinements = int(refined.value)
+            return rec, None, int(refined.value)
         idx = np.empty((nblk, nt, nb), dtype=np.int64)
         val = np.empty((nblk, nt, nb), dtype=np.complex64)
-        refined = ctypes.c_int64(0)
         if coarse_threshold is None:
             check(lib().mfh_run_series(self.handle, sp, ns, on_dev, _ptr(st), _ptr(ws), _ptr(we), nblk,
                                        int(t0), int(nt), int(binsize), float(threshold),

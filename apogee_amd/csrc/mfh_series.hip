@@ -182,10 +182,10 @@ namespace {
 // Shared body of the flat and hierarchical run_series entry points.
 long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int series_on_device, const size_t *start,
                 const size_t *win_start, const size_t *win_end, int nblocks, int t0, int nt, size_t binsize,
-                float threshold, float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
+                float threshold, float coarse_thr, int64_t *idx, void *val_c64, void *rec, int32_t *counts,
                 int64_t *n_refined) {
   if (nblocks < 1) return fail("run_series needs at least one block");
-  if (!series_c64 || !start || !win_start || !win_end || !idx || !val_c64) return fail("null argument");
+  if (!series_c64 || !start || !win_start || !win_end || (!rec && (!idx || !val_c64))) return fail("null argument");
   if (t0 < 0 || nt < 1 || t0 + nt > c->nt) return fail("templates sub-range out of bounds");
   if (binsize < 1) return fail("binsize must be >= 1");
   const size_t n = c->n;
@@ -267,7 +267,7 @@ long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int s
     }
     b0 += g;
   }
-  const long total = mfh::read_peaks(c, pb, cells, nb, idx, val_c64, counts);  // waits for the stream
+  const long total = mfh::read_peaks(c, pb, cells, nb, idx, val_c64, counts, rec);  // waits for the stream
   if (total < 0) return -1;
   wait_on_return.armed = false;
   c->prof_collect();
@@ -283,7 +283,17 @@ extern "C" long mfh_run_series(mfh_ctx *ctx, const void *series_c64, size_t nser
                                int32_t *counts) {
   if (!ctx) return fail("null context");
   return run_series(reinterpret_cast<Ctx *>(ctx), false, series_c64, nseries, series_on_device, start, win_start,
-                    win_end, nblocks, t0, nt, binsize, threshold, 0.f, idx, val_c64, counts, nullptr);
+                    win_end, nblocks, t0, nt, binsize, threshold, 0.f, idx, val_c64, nullptr, counts, nullptr);
+}
+
+extern "C" long mfh_run_series_rec(mfh_ctx *ctx, const void *series_c64, size_t nseries, int series_on_device,
+                                   const size_t *start, const size_t *win_start, const size_t *win_end,
+                                   int nblocks, int t0, int nt, size_t binsize, float threshold, void *rec,
+                                   int32_t *counts) {
+  if (!ctx) return fail("null context");
+  if (!rec) return fail("null argument");
+  return run_series(reinterpret_cast<Ctx *>(ctx), false, series_c64, nseries, series_on_device, start, win_start,
+                    win_end, nblocks, t0, nt, binsize, threshold, 0.f, nullptr, nullptr, rec, counts, nullptr);
 }
 
 extern "C" long mfh_hier_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, int series_on_device,
@@ -292,7 +302,20 @@ extern "C" long mfh_hier_run_series(mfh_ctx *ctx, const void *series_c64, size_t
                                     int64_t *idx, void *val_c64, int32_t *counts, int64_t *n_refined) {
   if (!ctx) return fail("null context");
   return run_series(reinterpret_cast<Ctx *>(ctx), true, series_c64, nseries, series_on_device, start, win_start,
-                    win_end, nblocks, t0, nt, binsize, threshold, coarse_thr, idx, val_c64, counts, n_refined);
+                    win_end, nblocks, t0, nt, binsize, threshold, coarse_thr, idx, val_c64, nullptr, counts,
+                    n_refined);
+}
+
+extern "C" long mfh_hier_run_series_rec(mfh_ctx *ctx, const void *series_c64, size_t nseries,
+                                        int series_on_device, const size_t *start, const size_t *win_start,
+                                        const size_t *win_end, int nblocks, int t0, int nt, size_t binsize,
+                                        float threshold, float coarse_thr, void *rec, int32_t *counts,
+                                        int64_t *n_refined) {
+  if (!ctx) return fail("null context");
+  if (!rec) return fail("null argument");
+  return run_series(reinterpret_cast<Ctx *>(ctx), true, series_c64, nseries, series_on_device, start, win_start,
+                    win_end, nblocks, t0, nt, binsize, threshold, coarse_thr, nullptr, nullptr, rec, counts,
+                    n_refined);
 }
 
 // Block spectra into data rows: the forward half of run_series on its own.

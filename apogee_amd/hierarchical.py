@@ -261,14 +261,15 @@ class HierarchicalFilter(MatchedFilter):
         self._trig += int(refined)
         return idx, val, cnt
 
-    def _dispatch_series(self, ser, layout, binsize, threshold, t0, nt):
+    def _dispatch_series(self, ser, layout, binsize, threshold, t0, nt, records=False):
         band, f, thr = self._calibration()
         self._apply_band(band, f)
-        idx, val, refined = self._gpu.run_series(ser, layout.starts, layout.low, layout.high, t0, nt,
-                                                 binsize, threshold, coarse_threshold=thr)
-        self._pairs += idx.shape[0] * idx.shape[1]
+        first, second, refined = self._gpu.run_series(ser, layout.starts, layout.low, layout.high, t0, nt,
+                                                      binsize, threshold, coarse_threshold=thr,
+                                                      records=PEAK_DTYPE if records else None)
+        self._pairs += first.shape[0] * first.shape[1]
         self._trig += int(refined)
-        return idx, val
+        return first, second
 
     def _on_ingest(self, kind, index, a):
         if kind == "template":

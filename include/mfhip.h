@@ -286,6 +286,15 @@ long mfh_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, int se
                     int t0, int nt, size_t binsize, float threshold,
                     int64_t *idx, void *val_c64, int32_t *counts);
 
+/* As mfh_run_series, with the results as 16-byte records (see
+ * mfh_flat_peaks_rec), [block][template][bin].
+ * Replaces: the record assembly of _format_result
+ * (python/matchedfilter/__init__.py:155-173) after the GPU run_series. */
+long mfh_run_series_rec(mfh_ctx *ctx, const void *series_c64, size_t nseries, int series_on_device,
+                        const size_t *start, const size_t *win_start, const size_t *win_end, int nblocks,
+                        int t0, int nt, size_t binsize, float threshold,
+                        void *rec, int32_t *counts);
+
 /* The block spectra of run_series on their own: spectrum b = FFT of
  * series[start[b] : start[b] + n] (zero-padded past the end) with the 1/n
  * applied first, written into data rows [row0, row0 + nblocks) of the
@@ -305,6 +314,16 @@ long mfh_hier_run_series(mfh_ctx *ctx, const void *series_c64, size_t nseries, i
                          int nblocks, int t0, int nt, size_t binsize, float threshold,
                          float coarse_thr, int64_t *idx, void *val_c64, int32_t *counts,
                          int64_t *n_refined);
+
+/* As mfh_hier_run_series, with the results as 16-byte records (see
+ * mfh_flat_peaks_rec).
+ * Replaces: the record assembly of _format_result
+ * (python/matchedfilter/__init__.py:155-173) after ap_hmf_run_series. */
+long mfh_hier_run_series_rec(mfh_ctx *ctx, const void *series_c64, size_t nseries, int series_on_device,
+                             const size_t *start, const size_t *win_start, const size_t *win_end,
+                             int nblocks, int t0, int nt, size_t binsize, float threshold,
+                             float coarse_thr, void *rec, int32_t *counts,
+                             int64_t *n_refined);
 
 /* Tuning knobs: key "dpw" (data rows per workgroup; 0 = automatic),
  * "refine_grid" (workgroups of a list-driven refine), "coarse_tpw"

@@ -72,12 +72,12 @@ class FakeContext:
         return self._as_records(idx, val.astype(np.complex64), records) + (cnt, int(fired.sum()))
 
     def run_series(self, series, starts, win_start, win_end, t0, nt, binsize, threshold,
-                   coarse_threshold=None):
+                   coarse_threshold=None, records=None):
         FakeContext.calls.append(("series", len(starts), t0, nt, binsize, threshold, coarse_threshold))
         T = self.T[t0:t0 + nt]
         idx, val, _ = O.series_peaks(np.asarray(series), starts, win_start, win_end, T, binsize,
                                      threshold)
-        return idx, val.astype(np.complex64), 0
+        return self._as_records(idx, val.astype(np.complex64), records) + (0,)
 
 
 class FakeQueue:

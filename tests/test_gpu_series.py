@@ -144,6 +144,13 @@ def test_hierarchical_series_is_identical_or_absent(gpu_device):
     np.testing.assert_array_equal(a["value"][fired], b["value"][fired])
     pairs, trig = hier.stats
     assert pairs == len(starts) * nt and trig < pairs
+    # the records the library fills (mfh_run_series_rec, mfh_hier_run_series_rec)
+    # equal the raw arrays of the same call bit for bit
+    for f, rec in ((flat, a), (hier, b)):
+        idx, val = f.run_series(series, starts, ws, we, binsize=512, threshold=5.0, raw=True)
+        assert rec.dtype == mf.PEAK_DTYPE and rec.shape == idx.shape
+        np.testing.assert_array_equal(rec["index"], idx)
+        np.testing.assert_array_equal(rec["value"], val)
 
 
 @pytest.mark.parametrize("nblk", [1, 4, 6])

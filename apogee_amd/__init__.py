@@ -329,37 +329,25 @@ class MatchedFilter:
     def _series_layout(self, series, starts, win_start, win_end, binsize, templates):
         """Check a run_series request (the reference's contract,
         python/matchedfilter/__init__.py:517-543) and build its layout."""
-        flat_err = "series, starts, win_start and win_end must be one-dimensional"
+        from ._series import SeriesLayout, offsets_table, _FLAT
         if _on_rocm(series):
             import torch
             ser = torch.from_dlpack(series) if not isinstance(series, torch.Tensor) else series
             if ser.dim() != 1:
-                raise ValueError(flat_err)
+                raise ValueError(_FLAT)
             ser = ser.to(torch.complex64).contiguous()
         else:
             ser = np.ascontiguousarray(_host_view(series), dtype=np.complex64)
             if ser.ndim != 1:
-                raise ValueError(flat_err)
-        offsets = [np.asarray(_host_view(a)) for a in (starts, win_start, win_end)]
-        if any(a.ndim != 1 for a in offsets):
-            raise ValueError(flat_err)
-        for a in offsets:
-            if np.issubdtype(a.dtype, np.signedinteger) and a.size and a.min() < 0:
-                raise ValueError("starts and windows must be nonnegative")
-        st, ws, we = (np.ascontiguousarray(a, dtype=np.uintp) for a in offsets)
-        if len({st.size, ws.size, we.size}) != 1:
-            raise ValueError("starts, win_start and win_end must be the same length")
-        if st.size == 0:
-            raise ValueError("run_series needs at least one block")
-        if (st > np.iinfo(np.intp).max - self.n).any():
-            raise ValueError("starts must be nonnegative and fit in the sample index range")
+                raise ValueError(_FLAT)
+        table, low, high = offsets_table([np.asarray(_host_view(a)) for a in (starts, win_start, win_end)],
+                                         self.n)
         width = _bin_width(binsize, self.n)
         t0, nt = _row_span(templates, self.ntemplates)
         if nt < 1 or t0 < 0 or t0 + nt > self.ntemplates:
             raise ValueError("templates sub-range out of bounds")
         self._require_templates(t0, nt)
-        from ._series import SeriesLayout
-        return ser, SeriesLayout(self.n, st, ws, we, width), width, t0, nt
+        return ser, SeriesLayout(self.n, table, low, high, width), width, t0, nt
 
     def run_series(self, series, starts, win_start, win_end,
                    binsize=None, threshold=0.0, templates=None, raw=False):

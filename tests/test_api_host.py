@@ -406,3 +406,51 @@ def test_noise_exceedance_model():
     # a narrower profile has fewer independent lags: it exceeds less often
     assert G.noise_exceedance(inspiral_power(n), n, 512, 4.0) < G.noise_exceedance(white, n, 512, 4.0)
     assert G.noise_exceedance(white, n, 512, 0.0) == 1.0
+
+
+def test_series_layout_table_matches_a_per_block_reading():
+    """The (3, nblocks) table check of run_series against a per-block reading
+    of the contract: clamped windows, one bin count, the errors and their
+    kinds, for uniform and mixed windows, windows past n and a binsize past n."""
+    from apogee_amd._series import SeriesLayout, offsets_table
+    rng = np.random.default_rng(31)
+    n = 1024
+    for trial in range(400):
+        m = int(rng.integers(1, 9))
+        st = rng.integers(0, 5000, m)
+        if trial % 2:
+            lo = np.full(m, int(rng.integers(0, n + 200)))
+            hi = np.full(m, int(rng.integers(0, n + 400)))
+        else:
+            lo = rng.integers(0, n + 200, m)
+            hi = lo + rng.integers(-50, 600, m)
+        binsize = int(rng.choice([1, 7, 64, 300, n, 10 * n, 1 << 70]))
+        dtype = [np.int64, np.uint64, np.int32][trial % 3]
+        rows = [np.asarray(a, dtype) for a in (st, lo, np.maximum(hi, 0))]
+        clo, chi = np.minimum(lo, n), np.minimum(np.maximum(hi, 0), n)
+        ok = (clo < chi).all() and len({1 + (int(b) - int(a) - 1) // binsize for a, b in zip(clo, chi)}) == 1
+        if not ok:
+            with pytest.raises(ValueError):
+                SeriesLayout(n, *offsets_table(rows, n), binsize)
+            continue
+        lay = SeriesLayout(n, *offsets_table(rows, n), binsize)
+        assert lay.starts.dtype == np.uintp and lay.low.dtype == np.uintp and lay.high.dtype == np.uintp
+        np.testing.assert_array_equal(lay.starts, st)
+        np.testing.assert_array_equal(lay.low, clo)
+        np.testing.assert_array_equal(lay.high, chi)
+        assert lay.nbins == 1 + (int(chi[0]) - int(clo[0]) - 1) // binsize
+    with pytest.raises(ValueError, match="one-dimensional"):
+        offsets_table([np.zeros((1, 1), np.int64), np.zeros(1, np.int64), np.ones(1, np.int64)], n)
+    with pytest.raises(ValueError, match="same length"):
+        offsets_table([np.zeros(2, np.int64), np.zeros(1, np.int64), np.ones(1, np.int64)], n)
+    with pytest.raises(ValueError, match="at least one block"):
+        offsets_table([np.zeros(0, np.int64)] * 3, n)
+    big = np.array([np.iinfo(np.uintp).max], np.uintp)
+    for rows, what in (([np.array([-1]), np.zeros(1, np.int64), np.ones(1, np.int64)], "starts"),
+                       ([big, np.zeros(1, np.int64), np.ones(1, np.int64)], "starts"),
+                       ([np.array([np.iinfo(np.intp).max - n + 1]), np.zeros(1, np.int64), np.ones(1, np.int64)],
+                        "starts"),
+                       ([np.zeros(1, np.int64), np.array([-3]), np.ones(1, np.int64)], "window"),
+                       ([np.zeros(1, np.int64), np.zeros(1, np.int64), big], "window")):
+        with pytest.raises(ValueError, match=what):
+            offsets_table(rows, n)

@@ -320,6 +320,13 @@ extern "C" int mfh_stream_wait(mfh_ctx *ctx, void *producer_stream) {
   if (!ctx) return fail("null context");
   Ctx *c = reinterpret_cast<Ctx *>(ctx);
   hipSetDevice(c->dev);
+  // a producer stream with nothing left to run has finished every write the
+  // caller ordered before this hand-over: no wait to enqueue (the cross-queue
+  // wait costs tens of microseconds of host time per call)
+  const hipError_t q = hipStreamQuery((hipStream_t)producer_stream);
+  if (q == hipSuccess) return 0;
+  if (q != hipErrorNotReady) return hip_ok(q, "hipStreamQuery") ? 0 : -1;
+  (void)hipGetLastError();  // not-ready is an answer, not an error for later checks
   if (!c->order_ev && !hip_ok(hipEventCreateWithFlags(&c->order_ev, hipEventDisableTiming), "hipEventCreate"))
     return -1;
   // a wait enqueued here holds the event's state at this call, so the event

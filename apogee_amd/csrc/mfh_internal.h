@@ -187,8 +187,17 @@ int launch_big(Ctx *c, size_t n, const FlatRun &r);
 // Four-step root tables for n = n1 * 256: w_n1^h (h < n1) then w_n^l (l < 256),
 // FP64-generated once per context.
 const cf *big_roots(Ctx *c, int n1);
+// Start offsets of run_series blocks as the forward kernels read them: up to
+// kInline in the kernel arguments themselves (no upload), else a device table.
+struct BlockStarts {
+  static constexpr int kInline = 64;
+  const long long *dev = nullptr;  // device table of every block, or null: `inl` holds them
+  long long inl[kInline];
+  __device__ long long operator[](int b) const { return dev ? dev[b] : inl[b]; }
+};
 // Forward spectra of run_series blocks into out [nblocks][n] (device).
-int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out);
+int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const BlockStarts &starts, int nblocks,
+                          cf *out);
 // MFMA coarse statistic (max |z_c|^2 over [cs, ce)) per pair, band 512;
 // `zero` (may be null) is set to 0 by the launch.
 int coarse_mfma_prepare(Ctx *c);

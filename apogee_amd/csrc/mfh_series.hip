@@ -25,7 +25,7 @@ namespace {
 // transform -- in natural order.
 template <int N>
 __global__ void __launch_bounds__(FlatShape<N>::WG) __attribute__((amdgpu_waves_per_eu(4)))
-series_forward_kernel(const cf *series, long long nseries, const long long *starts, int nblocks, const cf *tw,
+series_forward_kernel(const cf *series, long long nseries, const BlockStarts starts, int nblocks, const cf *tw,
                       cf *out) {
   using S = FlatShape<N>;
   constexpr int TPT = S::TPT;
@@ -53,7 +53,7 @@ series_forward_kernel(const cf *series, long long nseries, const long long *star
 }
 
 template <int N>
-int launch_fwd(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out) {
+int launch_fwd(Ctx *c, const cf *series, size_t nseries, const BlockStarts &starts, int nblocks, cf *out) {
   using S = FlatShape<N>;
   const cf *tw = c->twiddles(N);
   if (!tw) return -1;
@@ -71,15 +71,15 @@ int launch_fwd(Ctx *c, const cf *series, size_t nseries, const long long *starts
 // rows run N1-point inverse FFTs over f1 and store conj(z) at k2 + 256 k1.
 template <int N1>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-series_big_cols(const cf *series, long long nseries, const long long *starts, const cf *tw256, const cf *rootH,
-                const cf *rootL, cf *Y) {
+series_big_cols(const cf *series, long long nseries, const BlockStarts starts, int b0, const cf *tw256,
+                const cf *rootH, const cf *rootL, cf *Y) {
   constexpr int N = N1 * 256;
   extern __shared__ __align__(16) unsigned char smem[];
   const int s = threadIdx.x & 15;
   const int t = threadIdx.x >> 4;
   cf *xch = reinterpret_cast<cf *>(smem) + s * kColStride;
   const int b = blockIdx.y;
-  const long long s0 = starts[b];
+  const long long s0 = starts[b0 + b];
   const int f1 = blockIdx.x * 16 + s;
   const float inv = 1.0f / (float)N;
   cf v[16];
@@ -122,7 +122,7 @@ series_big_rows(const cf *Y, const cf *twN1, cf *out) {
 }
 
 template <int N1>
-int launch_fwd_big(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out) {
+int launch_fwd_big(Ctx *c, const cf *series, size_t nseries, const BlockStarts &starts, int nblocks, cf *out) {
   constexpr int N = N1 * 256;
   using S = FlatShape<N1>;
   const cf *tw256 = c->twiddles(256);
@@ -140,7 +140,7 @@ int launch_fwd_big(Ctx *c, const cf *series, size_t nseries, const long long *st
   for (int b0 = 0; b0 < nblocks; b0 += chunk) {
     const int nb = std::min(chunk, nblocks - b0);
     hipLaunchKernelGGL(series_big_cols<N1>, dim3(N1 / 16, nb), dim3(256), lds_cols, c->stream, series,
-                       (long long)nseries, starts + b0, tw256, roots, roots + N1, Y);
+                       (long long)nseries, starts, b0, tw256, roots, roots + N1, Y);
     hipLaunchKernelGGL(series_big_rows<N1>, dim3(256 / S::SLOTS, nb), dim3(256), lds_rows, c->stream, Y, twN1,
                        out + (long long)b0 * N);
   }
@@ -149,7 +149,8 @@ int launch_fwd_big(Ctx *c, const cf *series, size_t nseries, const long long *st
 
 }  // namespace
 
-int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const long long *starts, int nblocks, cf *out) {
+int launch_series_forward(Ctx *c, const cf *series, size_t nseries, const BlockStarts &starts, int nblocks,
+                          cf *out) {
   switch (c->n) {
     case 64: return launch_fwd<64>(c, series, nseries, starts, nblocks, out);
     case 128: return launch_fwd<128>(c, series, nseries, starts, nblocks, out);
@@ -179,6 +180,26 @@ using mfh::hip_ok;
 
 namespace {
 
+// The blocks' start offsets, clamped to nseries, as the forward kernels take
+// them: in the kernel arguments up to BlockStarts::kInline blocks, else copied
+// from `host` (which must outlive the queued copy) into a device table.
+bool block_starts(Ctx *c, const size_t *start, size_t nseries, int nblocks, mfh::BlockStarts &bs,
+                  std::vector<long long> &host) {
+  const auto clamp = [&](int b) { return start[b] < nseries ? (long long)start[b] : (long long)nseries; };
+  if (nblocks <= mfh::BlockStarts::kInline) {
+    bs.dev = nullptr;
+    for (int b = 0; b < nblocks; b++) bs.inl[b] = clamp(b);
+    return true;
+  }
+  long long *dstart = (long long *)c->scratch("series_starts", (size_t)nblocks * sizeof(long long));
+  if (!dstart) return false;
+  host.resize(nblocks);
+  for (int b = 0; b < nblocks; b++) host[b] = clamp(b);
+  bs.dev = dstart;
+  return hip_ok(hipMemcpyAsync(dstart, host.data(), nblocks * sizeof(long long), hipMemcpyHostToDevice, c->stream),
+                "copy starts");
+}
+
 // Shared body of the flat and hierarchical run_series entry points.
 long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int series_on_device, const size_t *start,
                 const size_t *win_start, const size_t *win_end, int nblocks, int t0, int nt, size_t binsize,
@@ -198,10 +219,9 @@ long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int s
   }
   hipSetDevice(c->dev);
   cf *spec = (cf *)c->scratch("series_spec", (size_t)nblocks * n * sizeof(cf));
-  long long *dstart = (long long *)c->scratch("series_starts", (size_t)nblocks * sizeof(long long));
   const size_t cells = (size_t)nblocks * nt * nb;
   mfh::PeakBuf pb;
-  if (!spec || !dstart || !mfh::peak_buffer(c, cells, pb)) return -1;
+  if (!spec || !mfh::peak_buffer(c, cells, pb)) return -1;
   int64_t *didx = pb.idx;
   cf *dval = pb.val;
   const cf *dser;
@@ -215,8 +235,7 @@ long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int s
       return -1;
     dser = buf;
   }
-  std::vector<long long> st(nblocks);
-  for (int b = 0; b < nblocks; b++) st[b] = start[b] < nseries ? (long long)start[b] : (long long)nseries;
+  std::vector<long long> st;
   // `st` must outlive its copy: the call ends with a stream synchronisation,
   // and an early return waits for the stream (this guard, declared after
   // `st`, is destroyed before it)
@@ -227,10 +246,9 @@ long run_series(Ctx *c, bool hier, const void *series_c64, size_t nseries, int s
       if (armed) hipStreamSynchronize(s);
     }
   } wait_on_return{c->stream};
-  if (!hip_ok(hipMemcpyAsync(dstart, st.data(), nblocks * sizeof(long long), hipMemcpyHostToDevice, c->stream),
-              "copy starts"))
-    return -1;
-  if (mfh::launch_series_forward(c, dser, nseries, dstart, nblocks, spec) < 0) return -1;
+  mfh::BlockStarts bs;
+  if (!block_starts(c, start, nseries, nblocks, bs, st)) return -1;
+  if (mfh::launch_series_forward(c, dser, nseries, bs, nblocks, spec) < 0) return -1;
   long long refined_total = 0;
   std::vector<int> refined;
   // consecutive blocks with equal windows share one launch
@@ -327,8 +345,6 @@ extern "C" int mfh_series_forward(mfh_ctx *ctx, const void *series_c64, size_t n
   if (!series_c64 || !start) return fail("null argument");
   if (row0 < 0 || row0 + nblocks > c->nd) return fail("data rows [%d, %d) out of bounds", row0, row0 + nblocks);
   hipSetDevice(c->dev);
-  long long *dstart = (long long *)c->scratch("series_starts", (size_t)nblocks * sizeof(long long));
-  if (!dstart) return -1;
   const cf *dser;
   if (series_on_device) {
     dser = (const cf *)series_c64;
@@ -340,12 +356,11 @@ extern "C" int mfh_series_forward(mfh_ctx *ctx, const void *series_c64, size_t n
       return -1;
     dser = buf;
   }
-  std::vector<long long> st(nblocks);
-  for (int b = 0; b < nblocks; b++) st[b] = start[b] < nseries ? (long long)start[b] : (long long)nseries;
-  if (!hip_ok(hipMemcpyAsync(dstart, st.data(), nblocks * sizeof(long long), hipMemcpyHostToDevice, c->stream),
-              "copy starts"))
-    return -1;
-  if (c->own_bank(false) < 0) return -1;
-  if (mfh::launch_series_forward(c, dser, nseries, dstart, nblocks, c->data + (size_t)row0 * c->n) < 0) return -1;
-  return hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize") ? 0 : -1;
+  std::vector<long long> st;
+  mfh::BlockStarts bs;
+  // every return below waits for the stream first: `st` must outlive its copy
+  const bool ok = block_starts(c, start, nseries, nblocks, bs, st) && c->own_bank(false) >= 0 &&
+                  mfh::launch_series_forward(c, dser, nseries, bs, nblocks, c->data + (size_t)row0 * c->n) >= 0;
+  const bool synced = hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return ok && synced ? 0 : -1;
 }

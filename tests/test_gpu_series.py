@@ -51,6 +51,27 @@ def test_half_overlap_real_series(n, gpu_device):
     check_against_oracle(series.astype(np.complex64), starts, ws, we, T, n // 8, 0.0, got)
 
 
+@pytest.mark.parametrize("n,nblk", [(1024, 64), (1024, 65), (256, 300), (1048576, 40)])
+def test_start_tables_inline_and_on_the_device(n, nblk, gpu_device):
+    """Up to 64 block starts travel in the forward kernels' arguments, more
+    go through a device table; at n = 2^20 the four-step split takes 32
+    blocks per chunk, so the inline starts are read past a chunk offset.
+    Against the oracle, starts unordered and some past the end."""
+    rng = np.random.default_rng(nblk)
+    nt = 2
+    nser = 3 * n + 1000
+    series = (rng.standard_normal(nser) + 1j * rng.standard_normal(nser)).astype(np.complex64)
+    T = (rng.standard_normal((nt, n)) + 1j * rng.standard_normal((nt, n))).astype(np.complex64)
+    starts = rng.integers(0, nser + n // 2, nblk).astype(np.uintp)
+    ws = np.full(nblk, n // 8, np.uintp)
+    we = np.full(nblk, n - n // 8, np.uintp)
+    f = mf.MatchedFilter(n, 1, nt, device="gpu")
+    f.set_templates(T)
+    got = f.run_series(series, starts, ws, we, binsize=n // 4, threshold=0.0)
+    assert got.shape == (nblk, nt, 3)
+    check_against_oracle(series, starts, ws, we, T, n // 4, 0.0, got)
+
+
 def test_ragged_edges_and_zero_padding(gpu_device):
     n, nt = 2048, 3
     rng = np.random.default_rng(2)

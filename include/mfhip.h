@@ -67,7 +67,8 @@ int mfh_sync(mfh_ctx *ctx);
 /* Order the context's stream after the work queued so far on
  * `producer_stream` (a hipStream_t of the context's device; null = the null
  * stream): an event recorded there that the context's stream waits on, with
- * no host wait.  The Python binding hands over PyTorch tensors this way,
+ * no host wait, or nothing when that stream has no work left to run.  The
+ * Python binding hands over PyTorch tensors this way,
  * which is what a DLPack __dlpack__(stream=...) exchange does, without the
  * capsule.  No counterpart in the reference, which refuses device arrays
  * (python/matchedfilter/__init__.py:103-131). */

@@ -76,7 +76,9 @@ void *Ctx::host_stage(size_t bytes) {
     stage = nullptr;
     stage_bytes = 0;
   }
-  if (!hip_ok(hipHostMalloc(&stage, bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc")) {
+  // coherent: the readback kernel's stores reach host memory without
+  // stopping in the GPU's caches
+  if (!hip_ok(hipHostMalloc(&stage, bytes ? bytes : 1, hipHostMallocCoherent), "hipHostMalloc")) {
     stage = nullptr;
     return nullptr;
   }
@@ -92,15 +94,47 @@ bool peak_buffer(Ctx *c, size_t cells, PeakBuf &b) {
   return true;
 }
 
+namespace {
+
+// A call's device results written straight into page-locked host memory in
+// the layout the caller asked for: 16-byte {int64 index; complex64 value}
+// records, or [indices | values].  A kernel queued behind the filter takes
+// the place of a copy-engine transfer, whose start and completion cost more
+// than moving these few hundred KiB.
+__global__ void __launch_bounds__(256) results_to_host(const int64_t *__restrict__ idx, const cf *__restrict__ val,
+                                                       long long cells, int records, char *__restrict__ dst) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= cells) return;
+  const long long k = idx[i];
+  const cf v = val[i];
+  if (records) {
+    int4 w;
+    w.x = (int)(unsigned)(unsigned long long)k;
+    w.y = (int)(k >> 32);
+    w.z = __float_as_int(v.x);
+    w.w = __float_as_int(v.y);
+    reinterpret_cast<int4 *>(dst)[i] = w;
+  } else {
+    reinterpret_cast<long long *>(dst)[i] = k;
+    reinterpret_cast<cf *>(dst + cells * sizeof(long long))[i] = v;
+  }
+}
+
+}  // namespace
+
 long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts,
                 void *rec) {
   const size_t bi = cells * sizeof(int64_t), bytes = bi + cells * sizeof(cf);
   char *st = (char *)c->host_stage(bytes);
   std::vector<char> pageable;
-  if (st) {
-    if (!hip_ok(hipMemcpyAsync(st, b.idx, bytes, hipMemcpyDeviceToHost, c->stream), "copy results") ||
-        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
-      return -1;
+  const bool staged = st != nullptr;  // the readback kernel fills the staging in the caller's layout
+  if (staged) {
+    if (cells) {
+      hipLaunchKernelGGL(results_to_host, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, c->stream, b.idx,
+                         b.val, (long long)cells, rec ? 1 : 0, st);
+      if (!hip_ok(hipGetLastError(), "results_to_host")) return -1;
+    }
+    if (!hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return -1;
   } else if (!rec) {
     // no page-locked staging to be had: copy straight into the caller's
     // (pageable) arrays, one copy each
@@ -118,9 +152,11 @@ long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx,
         !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
       return -1;
   }
+  // the indices to count, `step` int64 words apart
   const int64_t *si = st ? reinterpret_cast<const int64_t *>(st) : idx;
-  if (rec) {
-    // one pass from the staging copy into {int64 index; complex64 value}
+  size_t step = 1;
+  if (rec && !staged) {
+    // one pass from the pageable copy into {int64 index; complex64 value}
     // records (16 bytes each)
     const cf *sv = reinterpret_cast<const cf *>(st + bi);
     char *r = static_cast<char *>(rec);
@@ -128,7 +164,14 @@ long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx,
       memcpy(r + 16 * i, si + i, 8);
       memcpy(r + 16 * i + 8, sv + i, 8);
     }
-  } else if (st) {
+    si = reinterpret_cast<const int64_t *>(rec);
+    step = 2;
+  } else if (rec) {
+    // the staging already holds the records
+    if (cells) memcpy(rec, st, cells * 16);
+    si = reinterpret_cast<const int64_t *>(rec);
+    step = 2;
+  } else if (staged && cells) {
     memcpy(idx, st, bi);
     memcpy(val_c64, st + bi, bytes - bi);
     si = idx;
@@ -136,7 +179,7 @@ long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx,
   long total = 0;
   for (size_t p = 0; p < cells / nb; p++) {
     int cnt = 0;
-    for (size_t k = 0; k < nb; k++) cnt += si[p * nb + k] >= 0;
+    for (size_t k = 0; k < nb; k++) cnt += si[(p * nb + k) * step] >= 0;
     if (counts) counts[p] = cnt;
     total += cnt;
   }

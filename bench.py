@@ -570,8 +570,10 @@ def progress(leg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    # 1000 steps: about 0.1 s timed, so that a millisecond of host or clock
+    # disturbance moves the figure by about 1%, not the 15% it did at 200
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-probes", action="store_true", help="skip the measured VALU / copy ceilings")
     ap.add_argument("--no-hier", action="store_true", help="skip the hierarchical leg")

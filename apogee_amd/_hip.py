@@ -69,6 +69,9 @@ def _declare(lib):
     lib.mfh_hier_peaks_async.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          _c_size, ctypes.c_float, _c_size, _c_size, ctypes.c_float,
                                          _vp, _vp, _vp]
+    lib.mfh_host_alloc.restype = _vp
+    lib.mfh_host_alloc.argtypes = [_c_size]
+    lib.mfh_host_free.argtypes = [_vp]
     for name in ("mfh_run_series", "mfh_hier_run_series", "mfh_run_series_rec", "mfh_hier_run_series_rec"):
         getattr(lib, name).restype = ctypes.c_long
     lib.mfh_run_series.argtypes = [_vp, _vp, _c_size, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int,
@@ -155,6 +158,67 @@ def lib():
 
 def last_error():
     return lib().mfh_last_error().decode(errors="replace")
+
+
+class _HostBlock:
+    """One page-locked buffer of mfh_host_alloc, back to the pool when the
+    last array over it is gone."""
+
+    __slots__ = ("ptr", "nbytes")
+
+    def __init__(self, ptr, nbytes):
+        self.ptr, self.nbytes = ptr, nbytes
+
+    def __del__(self):
+        try:
+            _POOL.give(self.ptr, self.nbytes)
+        except Exception:       # interpreter shutdown: the process frees it
+            pass
+
+
+class _HostPool:
+    """Free page-locked result buffers by size.  A records array handed to a
+    caller owns its buffer until the array is gone, so results never alias;
+    the pool keeps at most `keep_bytes` of free buffers."""
+
+    def __init__(self, keep_bytes=256 << 20):
+        self.free = {}
+        self.free_bytes = 0
+        self.keep_bytes = keep_bytes
+        self.lock = threading.Lock()
+
+    def take(self, nbytes):
+        with self.lock:
+            got = self.free.get(nbytes)
+            if got:
+                self.free_bytes -= nbytes
+                return got.pop()
+        ptr = lib().mfh_host_alloc(nbytes)
+        return int(ptr) if ptr else None
+
+    def give(self, ptr, nbytes):
+        with self.lock:
+            if self.free_bytes + nbytes <= self.keep_bytes:
+                self.free.setdefault(nbytes, []).append(ptr)
+                self.free_bytes += nbytes
+                return
+        lib().mfh_host_free(ctypes.c_void_p(ptr))
+
+
+_POOL = _HostPool()
+
+
+def _records_array(shape, dtype):
+    """An empty records array in page-locked memory the device writes
+    directly (a plain NumPy array when none can be had)."""
+    count = int(np.prod(shape, dtype=np.int64))
+    nbytes = max(1, count * dtype.itemsize)
+    ptr = _POOL.take(nbytes)
+    if ptr is None:
+        return np.empty(shape, dtype=dtype)
+    buf = (ctypes.c_char * nbytes).from_address(ptr)
+    buf._block = _HostBlock(ptr, nbytes)
+    return np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
 
 
 def _record_dtype(dtype):
@@ -441,7 +505,7 @@ class Context:
         args = (self.handle, int(d0), int(nd), int(t0), int(nt), int(binsize), float(threshold), int(start),
                 int(end))
         if records is not None:
-            rec = np.empty((nd, nt, nb), dtype=_record_dtype(records))
+            rec = _records_array((nd, nt, nb), _record_dtype(records))
             check(lib().mfh_flat_peaks_rec(*args, _ptr(rec), _ptr(cnt) if cnt is not None else None),
                   "mfh_flat_peaks_rec")
             return rec, None, cnt
@@ -474,7 +538,7 @@ class Context:
         args = (self.handle, int(d0), int(nd), int(t0), int(nt), int(binsize), float(threshold), int(start),
                 int(end), float(coarse_threshold))
         if records is not None:
-            rec = np.empty((nd, nt, nb), dtype=_record_dtype(records))
+            rec = _records_array((nd, nt, nb), _record_dtype(records))
             check(lib().mfh_hier_peaks_rec(*args, _ptr(rec), _ptr(cnt) if cnt is not None else None,
                                            ctypes.byref(refined)), "mfh_hier_peaks_rec")
             first, second = rec, None
@@ -525,7 +589,7 @@ class Context:
         nb = int(lib().mfh_nbins(self.n, int(binsize), int(ws[0]), int(we[0])))
         refined = ctypes.c_int64(0)
         if records is not None:
-            rec = np.empty((nblk, nt, nb), dtype=_record_dtype(records))
+            rec = _records_array((nblk, nt, nb), _record_dtype(records))
             if coarse_threshold is None:
                 check(lib().mfh_run_series_rec(self.handle, sp, ns, on_dev, _ptr(st), _ptr(ws), _ptr(we), nblk,
                                                int(t0), int(nt), int(binsize), float(threshold), _ptr(rec),

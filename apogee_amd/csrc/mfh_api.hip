@@ -122,9 +122,52 @@ __global__ void __launch_bounds__(256) results_to_host(const int64_t *__restrict
 
 }  // namespace
 
+// Bins set (index >= 0) among `cells` indices `step` int64 words apart;
+// counts[p] (if set) per pair of nb bins.  Flat loops the compiler
+// vectorises: a loop nest over the run-time bin count measured about 10 us
+// per 16384 cells on the host.
+long count_set(const int64_t *si, size_t step, size_t cells, size_t nb, int32_t *counts) {
+  long total = 0;
+  for (size_t i = 0; i < cells; i++) total += si[i * step] >= 0;
+  if (counts && nb == 1) {
+    for (size_t p = 0; p < cells; p++) counts[p] = si[p * step] >= 0;
+  } else if (counts) {
+    for (size_t p = 0; p < cells / nb; p++) {
+      int cnt = 0;
+      for (size_t k = 0; k < nb; k++) cnt += si[(p * nb + k) * step] >= 0;
+      counts[p] = cnt;
+    }
+  }
+  return total;
+}
+
+// Page-locked host buffers handed out by mfh_host_alloc: start -> bytes.
+std::mutex host_blocks_mu;
+std::map<uintptr_t, size_t> host_blocks;
+
+// Whether [p, p + bytes) lies inside one mfh_host_alloc buffer, which the
+// readback kernel may then write directly.
+bool host_block(const void *p, size_t bytes) {
+  const uintptr_t a = (uintptr_t)p;
+  std::lock_guard<std::mutex> lk(host_blocks_mu);
+  auto it = host_blocks.upper_bound(a);
+  if (it == host_blocks.begin()) return false;
+  --it;
+  return a - it->first + bytes <= it->second;
+}
+
 long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts,
                 void *rec) {
   const size_t bi = cells * sizeof(int64_t), bytes = bi + cells * sizeof(cf);
+  if (rec && cells && host_block(rec, cells * 16)) {
+    // the records go straight into the caller's page-locked buffer
+    hipLaunchKernelGGL(results_to_host, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, c->stream, b.idx,
+                       b.val, (long long)cells, 1, (char *)rec);
+    if (!hip_ok(hipGetLastError(), "results_to_host") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+      return -1;
+    return count_set(reinterpret_cast<const int64_t *>(rec), 2, cells, nb, counts);
+  }
   char *st = (char *)c->host_stage(bytes);
   std::vector<char> pageable;
   const bool staged = st != nullptr;  // the readback kernel fills the staging in the caller's layout
@@ -176,20 +219,7 @@ long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx,
     memcpy(val_c64, st + bi, bytes - bi);
     si = idx;
   }
-  // flat loops the compiler vectorises (a loop nest over a run-time bin
-  // count measured about 10 us per 16384 cells on the host)
-  long total = 0;
-  for (size_t i = 0; i < cells; i++) total += si[i * step] >= 0;
-  if (counts && nb == 1) {
-    for (size_t p = 0; p < cells; p++) counts[p] = si[p * step] >= 0;
-  } else if (counts) {
-    for (size_t p = 0; p < cells / nb; p++) {
-      int cnt = 0;
-      for (size_t k = 0; k < nb; k++) cnt += si[(p * nb + k) * step] >= 0;
-      counts[p] = cnt;
-    }
-  }
-  return total;
+  return count_set(si, step, cells, nb, counts);
 }
 
 const cf *Ctx::twiddles(size_t len) {
@@ -356,6 +386,28 @@ extern "C" void mfh_destroy(mfh_ctx *ctx) { delete reinterpret_cast<Ctx *>(ctx);
 
 extern "C" void *mfh_stream(mfh_ctx *ctx) {
   return ctx ? (void *)reinterpret_cast<Ctx *>(ctx)->stream : nullptr;
+}
+
+extern "C" void *mfh_host_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (!hip_ok(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocCoherent), "hipHostMalloc")) {
+    (void)hipGetLastError();  // a later launch check must not see this failure
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(mfh::host_blocks_mu);
+  mfh::host_blocks[(uintptr_t)p] = bytes ? bytes : 1;
+  return p;
+}
+
+extern "C" int mfh_host_free(void *p) {
+  if (!p) return 0;
+  {
+    std::lock_guard<std::mutex> lk(mfh::host_blocks_mu);
+    auto it = mfh::host_blocks.find((uintptr_t)p);
+    if (it == mfh::host_blocks.end()) return fail("not a buffer of mfh_host_alloc");
+    mfh::host_blocks.erase(it);
+  }
+  return hip_ok(hipHostFree(p), "hipHostFree") ? 0 : -1;
 }
 
 extern "C" int mfh_sync(mfh_ctx *ctx) {

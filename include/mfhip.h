@@ -64,6 +64,16 @@ void *mfh_stream(mfh_ctx *ctx);
 /* Wait for all work queued on the context's stream. */
 int mfh_sync(mfh_ctx *ctx);
 
+/* Page-locked host memory the device writes without caching it (coherent),
+ * for result records: a records array (the *_rec entries' `rec`) that lies
+ * inside such a buffer is written directly by the device, with no host
+ * copy.  mfh_host_alloc returns NULL on failure (message set);
+ * mfh_host_free returns 0 or < 0 (also for a pointer it did not hand out).
+ * Nearest reference counterpart: the host-visible shared arrays of
+ * empty_shared (python/matchedfilter/_shared.py:58-76). */
+void *mfh_host_alloc(size_t bytes);
+int mfh_host_free(void *p);
+
 /* Order the context's stream after the work queued so far on
  * `producer_stream` (a hipStream_t of the context's device; null = the null
  * stream): an event recorded there that the context's stream waits on, with

@@ -559,6 +559,39 @@ def test_lengths_beyond_the_reference_gpu_set(n, gpu_device):
                            context="n=%d binsize=%d" % (n, binsize))
 
 
+def test_records_own_their_page_locked_buffers(gpu_device):
+    """run() records come from pooled page-locked buffers the device writes
+    directly (mfh_host_alloc): a result held across calls keeps its values,
+    a freed result's buffer serves the next call, the values equal the raw
+    arrays, and mfh_host_free refuses memory it did not hand out."""
+    import ctypes
+    import gc
+    from apogee_amd import _hip
+    rng = np.random.default_rng(78)
+    n, nd, nt = 1024, 2, 9
+    D1, D2, T = cnoise(rng, (nd, n)), cnoise(rng, (nd, n)), cnoise(rng, (nt, n))
+    f = mf.MatchedFilter(n, nd, nt, device="gpu")
+    f.set_templates(T)
+    f.set_data(D1)
+    a = f.run(binsize=256)
+    held = a.copy()
+    f.set_data(D2)
+    b = f.run(binsize=256)
+    np.testing.assert_array_equal(a, held)          # the second call wrote elsewhere
+    assert not np.array_equal(a["index"], b["index"])
+    idx, val = f.run(binsize=256, raw=True)
+    np.testing.assert_array_equal(b["index"], idx)
+    np.testing.assert_array_equal(b["value"], val)
+    p = b.ctypes.data
+    del b
+    gc.collect()
+    c = f.run(binsize=256)
+    assert c.ctypes.data == p                       # the freed buffer, reused
+    np.testing.assert_array_equal(c["index"], idx)
+    np.testing.assert_array_equal(c["value"], val)
+    assert _hip.lib().mfh_host_free(ctypes.c_void_p(idx.ctypes.data)) < 0
+
+
 def test_records_equal_the_raw_arrays(gpu_device):
     """run() has the library fill PEAK_DTYPE records directly
     (mfh_flat_peaks_rec, mfh_hier_peaks_rec); they must equal the raw

@@ -221,6 +221,22 @@ def _records_array(shape, dtype):
     return np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
 
 
+def _raw_arrays(shape):
+    """Empty (index int64, value complex64) arrays back to back in one
+    page-locked buffer the device writes directly (plain NumPy arrays when
+    none can be had)."""
+    count = int(np.prod(shape, dtype=np.int64))
+    nbytes = max(1, count * 16)
+    ptr = _POOL.take(nbytes)
+    if ptr is None:
+        return np.empty(shape, dtype=np.int64), np.empty(shape, dtype=np.complex64)
+    buf = (ctypes.c_char * nbytes).from_address(ptr)
+    buf._block = _HostBlock(ptr, nbytes)
+    whole = np.frombuffer(buf, dtype=np.uint8, count=count * 16)
+    return (whole[:count * 8].view(np.int64).reshape(shape),
+            whole[count * 8:].view(np.complex64).reshape(shape))
+
+
 def _record_dtype(dtype):
     """The records layout the library writes: an int64 at byte 0 and a
     complex64 at byte 8, 16 bytes in all (little-endian)."""
@@ -509,8 +525,7 @@ class Context:
             check(lib().mfh_flat_peaks_rec(*args, _ptr(rec), _ptr(cnt) if cnt is not None else None),
                   "mfh_flat_peaks_rec")
             return rec, None, cnt
-        idx = np.empty((nd, nt, nb), dtype=np.int64)
-        val = np.empty((nd, nt, nb), dtype=np.complex64)
+        idx, val = _raw_arrays((nd, nt, nb))
         check(lib().mfh_flat_peaks(*args, _ptr(idx), _ptr(val), _ptr(cnt) if cnt is not None else None),
               "mfh_flat_peaks")
         return idx, val, cnt
@@ -543,8 +558,7 @@ class Context:
                                            ctypes.byref(refined)), "mfh_hier_peaks_rec")
             first, second = rec, None
         else:
-            idx = np.empty((nd, nt, nb), dtype=np.int64)
-            val = np.empty((nd, nt, nb), dtype=np.complex64)
+            idx, val = _raw_arrays((nd, nt, nb))
             check(lib().mfh_hier_peaks(*args, _ptr(idx), _ptr(val), _ptr(cnt) if cnt is not None else None,
                                        ctypes.byref(refined)), "mfh_hier_peaks")
             first, second = idx, val
@@ -601,8 +615,7 @@ class Context:
                                                     ctypes.byref(refined)), "mfh_hier_run_series_rec")
             self.last_refinements = int(refined.value)
             return rec, None, int(refined.value)
-        idx = np.empty((nblk, nt, nb), dtype=np.int64)
-        val = np.empty((nblk, nt, nb), dtype=np.complex64)
+        idx, val = _raw_arrays((nblk, nt, nb))
         if coarse_threshold is None:
             check(lib().mfh_run_series(self.handle, sp, ns, on_dev, _ptr(st), _ptr(ws), _ptr(we), nblk,
                                        int(t0), int(nt), int(binsize), float(threshold),

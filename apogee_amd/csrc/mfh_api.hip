@@ -168,6 +168,15 @@ long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx,
       return -1;
     return count_set(reinterpret_cast<const int64_t *>(rec), 2, cells, nb, counts);
   }
+  if (!rec && cells && idx && (char *)val_c64 == (char *)idx + bi && host_block(idx, bytes)) {
+    // [indices | values] straight into the caller's page-locked buffer
+    hipLaunchKernelGGL(results_to_host, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, c->stream, b.idx,
+                       b.val, (long long)cells, 0, (char *)idx);
+    if (!hip_ok(hipGetLastError(), "results_to_host") ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+      return -1;
+    return count_set(idx, 1, cells, nb, counts);
+  }
   char *st = (char *)c->host_stage(bytes);
   std::vector<char> pageable;
   const bool staged = st != nullptr;  // the readback kernel fills the staging in the caller's layout

@@ -65,8 +65,9 @@ void *mfh_stream(mfh_ctx *ctx);
 int mfh_sync(mfh_ctx *ctx);
 
 /* Page-locked host memory the device writes without caching it (coherent),
- * for result records: a records array (the *_rec entries' `rec`) that lies
- * inside such a buffer is written directly by the device, with no host
+ * for results: a records array (the *_rec entries' `rec`), or index and
+ * value arrays laid out back to back (values right after the indices), that
+ * lie inside such a buffer are written directly by the device, with no host
  * copy.  mfh_host_alloc returns NULL on failure (message set);
  * mfh_host_free returns 0 or < 0 (also for a pointer it did not hand out).
  * Nearest reference counterpart: the host-visible shared arrays of

@@ -563,7 +563,8 @@ def test_records_own_their_page_locked_buffers(gpu_device):
     """run() records come from pooled page-locked buffers the device writes
     directly (mfh_host_alloc): a result held across calls keeps its values,
     a freed result's buffer serves the next call, the values equal the raw
-    arrays, and mfh_host_free refuses memory it did not hand out."""
+    arrays (themselves one pooled buffer), and mfh_host_free refuses memory
+    it did not hand out."""
     import ctypes
     import gc
     from apogee_amd import _hip
@@ -589,7 +590,12 @@ def test_records_own_their_page_locked_buffers(gpu_device):
     assert c.ctypes.data == p                       # the freed buffer, reused
     np.testing.assert_array_equal(c["index"], idx)
     np.testing.assert_array_equal(c["value"], val)
-    assert _hip.lib().mfh_host_free(ctypes.c_void_p(idx.ctypes.data)) < 0
+    # raw results: indices and values back to back in one pooled buffer
+    idx2, val2 = f.run(binsize=256, raw=True)
+    assert val2.ctypes.data == idx2.ctypes.data + idx2.nbytes
+    np.testing.assert_array_equal(idx2, idx)
+    np.testing.assert_array_equal(val2, val)
+    assert _hip.lib().mfh_host_free(ctypes.c_void_p(np.empty(4).ctypes.data)) < 0
 
 
 def test_records_equal_the_raw_arrays(gpu_device):

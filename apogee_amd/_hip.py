@@ -206,6 +206,9 @@ class _HostPool:
 
 
 _POOL = _HostPool()
+# larger results go through the context's one staging buffer and a host copy,
+# so that each live result does not pin memory of its own beyond this
+_POOL_MAX_BYTES = 64 << 20
 
 
 def _records_array(shape, dtype):
@@ -213,7 +216,7 @@ def _records_array(shape, dtype):
     directly (a plain NumPy array when none can be had)."""
     count = int(np.prod(shape, dtype=np.int64))
     nbytes = max(1, count * dtype.itemsize)
-    ptr = _POOL.take(nbytes)
+    ptr = _POOL.take(nbytes) if nbytes <= _POOL_MAX_BYTES else None
     if ptr is None:
         return np.empty(shape, dtype=dtype)
     buf = (ctypes.c_char * nbytes).from_address(ptr)
@@ -227,7 +230,7 @@ def _raw_arrays(shape):
     none can be had)."""
     count = int(np.prod(shape, dtype=np.int64))
     nbytes = max(1, count * 16)
-    ptr = _POOL.take(nbytes)
+    ptr = _POOL.take(nbytes) if nbytes <= _POOL_MAX_BYTES else None
     if ptr is None:
         return np.empty(shape, dtype=np.int64), np.empty(shape, dtype=np.complex64)
     buf = (ctypes.c_char * nbytes).from_address(ptr)

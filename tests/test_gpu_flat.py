@@ -559,7 +559,7 @@ def test_lengths_beyond_the_reference_gpu_set(n, gpu_device):
                            context="n=%d binsize=%d" % (n, binsize))
 
 
-def test_records_own_their_page_locked_buffers(gpu_device):
+def test_records_own_their_page_locked_buffers(gpu_device, monkeypatch):
     """run() records come from pooled page-locked buffers the device writes
     directly (mfh_host_alloc): a result held across calls keeps its values,
     a freed result's buffer serves the next call, the values equal the raw
@@ -596,6 +596,15 @@ def test_records_own_their_page_locked_buffers(gpu_device):
     np.testing.assert_array_equal(idx2, idx)
     np.testing.assert_array_equal(val2, val)
     assert _hip.lib().mfh_host_free(ctypes.c_void_p(np.empty(4).ctypes.data)) < 0
+    # results above the pool's size limit: plain arrays through the staging copy
+    monkeypatch.setattr(_hip, "_POOL_MAX_BYTES", 64)
+    d = f.run(binsize=256)
+    idx3, val3 = f.run(binsize=256, raw=True)
+    assert val3.ctypes.data != idx3.ctypes.data + idx3.nbytes
+    np.testing.assert_array_equal(d["index"], idx)
+    np.testing.assert_array_equal(d["value"], val)
+    np.testing.assert_array_equal(idx3, idx)
+    np.testing.assert_array_equal(val3, val)
 
 
 def test_records_equal_the_raw_arrays(gpu_device):

@@ -138,12 +138,15 @@ struct PeakBuf {
   cf *val = nullptr;
 };
 bool peak_buffer(Ctx *c, size_t cells, PeakBuf &b);
-// Copy `cells` results into the caller's host arrays -- one device-to-host
-// copy into the pinned staging buffer, then host copies, or with `rec` one
-// pass writing 16-byte {int64 index; complex64 value} records there (idx and
-// val unused) -- after waiting for the context's stream; counts[p] (if set) =
-// bins set of pair p, nb bins per pair.  Returns the total of set bins, or -1
-// (message set).
+// Bring `cells` results to the caller's host arrays: a kernel queued on the
+// context's stream writes them in the caller's layout -- 16-byte {int64
+// index; complex64 value} records into `rec` (idx and val unused), else
+// [indices | values] -- straight into the caller's memory when it lies in an
+// mfh_host_alloc buffer (the values right after the indices, for idx/val),
+// else into the page-locked staging buffer followed by host copies (or
+// copy-engine transfers when no staging can be had); then waits for the
+// stream.  counts[p] (if set) = bins set of pair p, nb bins per pair.
+// Returns the total of set bins, or -1 (message set).
 long read_peaks(Ctx *c, const PeakBuf &b, size_t cells, size_t nb, int64_t *idx, void *val_c64, int32_t *counts,
                 void *rec = nullptr);
 

@@ -149,7 +149,9 @@ long mfh_flat_peaks(mfh_ctx *ctx, int d0, int nd, int t0, int nt,
 
 /* As mfh_flat_peaks, with the results as records: rec receives nd*nt*nbins
  * records {int64 index; float re, im} (16 bytes each, the PEAK_DTYPE layout
- * of the Python surface), filled in one pass over the page-locked readback.
+ * of the Python surface), written by the device in that layout: straight
+ * into rec when it lies in an mfh_host_alloc buffer, else into page-locked
+ * staging and then copied.
  * Replaces: the record assembly of _format_result
  * (python/matchedfilter/__init__.py:155-173) after Context.peaks. */
 long mfh_flat_peaks_rec(mfh_ctx *ctx, int d0, int nd, int t0, int nt,

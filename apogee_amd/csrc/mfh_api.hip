@@ -570,6 +570,16 @@ static long flat_peaks_host(mfh_ctx *ctx, int d0, int nd, int t0, int nt, size_t
   if (nb == 0) return fail("empty window (%zu, %zu)", ws, we);
   const size_t cells = (size_t)nd * (size_t)nt * nb;
   hipSetDevice(c->dev);
+  if (!rec && cells && (char *)val_c64 == (char *)idx + cells * sizeof(int64_t) &&
+      mfh::host_block(idx, cells * (sizeof(int64_t) + sizeof(cf)))) {
+    // [indices | values] in one page-locked mfh_host_alloc buffer: the
+    // filter writes the caller's arrays itself, with no readback step
+    if (mfh_flat_peaks_async(ctx, d0, nd, t0, nt, binsize, threshold, ws, we, idx, val_c64) < 0 ||
+        !hip_ok(hipStreamSynchronize(c->stream), "hipStreamSynchronize"))
+      return -1;
+    c->prof_collect();
+    return mfh::count_set(idx, 1, cells, nb, counts);
+  }
   mfh::PeakBuf pb;
   if (!mfh::peak_buffer(c, cells, pb)) return -1;
   if (mfh_flat_peaks_async(ctx, d0, nd, t0, nt, binsize, threshold, ws, we, pb.idx, pb.val) < 0) return -1;

@@ -607,6 +607,30 @@ def test_records_own_their_page_locked_buffers(gpu_device, monkeypatch):
     np.testing.assert_array_equal(val3, val)
 
 
+@pytest.mark.parametrize("n", [4096, 32768, 65536, 262144])
+def test_raw_results_written_in_place_equal_the_records(n, gpu_device):
+    """raw=True results are written into their pooled page-locked buffer by
+    the filter itself, records through the readback kernel: the two agree
+    bit for bit at one-pass and four-step lengths, with a window, ragged
+    bins and a threshold, counts included."""
+    rng = np.random.default_rng(n + 5)
+    nd, nt = 2, 3
+    D, T = cnoise(rng, (nd, n)), cnoise(rng, (nt, n))
+    D[1] += 3.0 * T[2]
+    f = mf.MatchedFilter(n, nd, nt, device="gpu")
+    f.set_data(D)
+    f.set_templates(T)
+    for kw in ({"binsize": n // 8 + 3, "threshold": 0.0, "window": (7, n - 5)},
+               {"binsize": n, "threshold": 4.0}):
+        rec, cnt = f.run(counts=True, **kw)
+        (idx, val), cnt_raw = f.run(counts=True, raw=True, **kw)
+        assert val.ctypes.data == idx.ctypes.data + idx.nbytes
+        np.testing.assert_array_equal(rec["index"], idx, err_msg=str(kw))
+        np.testing.assert_array_equal(rec["value"], val, err_msg=str(kw))
+        np.testing.assert_array_equal(cnt, cnt_raw, err_msg=str(kw))
+        np.testing.assert_array_equal(cnt, (idx >= 0).sum(axis=2), err_msg=str(kw))
+
+
 def test_records_equal_the_raw_arrays(gpu_device):
     """run() has the library fill PEAK_DTYPE records directly
     (mfh_flat_peaks_rec, mfh_hier_peaks_rec); they must equal the raw

@@ -185,7 +185,9 @@ class _HostPool:
         self.free = {}
         self.free_bytes = 0
         self.keep_bytes = keep_bytes
-        self.lock = threading.Lock()
+        # reentrant: a collection inside give() (it allocates) may free a
+        # result and call give() again on the same thread
+        self.lock = threading.RLock()
 
     def take(self, nbytes):
         with self.lock:

@@ -454,3 +454,35 @@ def test_series_layout_table_matches_a_per_block_reading():
                        ([np.zeros(1, np.int64), np.zeros(1, np.int64), big], "window")):
         with pytest.raises(ValueError, match=what):
             offsets_table(rows, n)
+
+
+def test_result_pool_reuses_and_caps_free_buffers(monkeypatch):
+    """The page-locked result pool's bookkeeping, against a stub library:
+    a returned buffer serves the next request of its size, free buffers
+    beyond the cap go back to the library, and a failed allocation gives
+    None (the caller then uses plain NumPy arrays)."""
+    from apogee_amd import _hip
+    freed, addrs = [], iter(range(1 << 20, 1 << 30, 1 << 16))
+
+    class Stub:
+        fail = False
+
+        def mfh_host_alloc(self, nbytes):
+            return None if Stub.fail else next(addrs)
+
+        def mfh_host_free(self, p):
+            freed.append(p.value)
+            return 0
+
+    monkeypatch.setattr(_hip, "lib", lambda: Stub())
+    pool = _hip._HostPool(keep_bytes=100)
+    a, b = pool.take(64), pool.take(64)
+    assert a != b
+    pool.give(a, 64)
+    assert pool.take(64) == a and pool.free_bytes == 0
+    pool.give(a, 64)
+    pool.give(b, 64)                      # 128 free bytes would pass the cap
+    assert freed == [b] and pool.free_bytes == 64
+    assert pool.take(32) not in (a, b)    # another size: a fresh buffer
+    Stub.fail = True
+    assert pool.take(48) is None

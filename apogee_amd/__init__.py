@@ -311,7 +311,10 @@ class MatchedFilter:
         ``raw=True`` the plain ``(index, value)`` arrays.  ``data`` /
         ``templates`` = ``(start, count)`` restrict the run to a sub-range
         (identical to the matching slice of a full run); ``window=(start,
-        end)`` restricts the lags searched.
+        end)`` restricts the lags searched.  Each result owns its storage
+        (up to 64 MiB, page-locked host memory the device writes directly),
+        so a later call never overwrites it, unlike the reference's reused
+        output buffer (python/matchedfilter/__init__.py:463-470).
         """
         req = self._check_run(binsize, window, data, templates)
         width, lo, hi, d0, nd, t0, nt = req
